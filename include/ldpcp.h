@@ -1,0 +1,185 @@
+/*
+ * ldpcp.h -- C ABI of the MI355X Langevin hot-path library (libldpcp.so).
+ *
+ * The reference has no FFI: its runner drives the integrator object
+ * LANGEVINEQUATION::LangevinEquationSIMDMultiple through C++ calls.  Each
+ * entry point below names the reference interface it replaces.  All entry
+ * points return 0 on success and a negative code on failure; the message is
+ * then available from ldpcp_last_error() (thread-local).  Outputs are caller
+ * allocated.  One handle serves one host thread; handles are independent and
+ * may run concurrently on separate HIP streams.
+ */
+#ifndef LDPCP_H
+#define LDPCP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LDPCP_MAX_CG 16
+
+#define LDPCP_OK 0
+#define LDPCP_EINVAL (-1)
+#define LDPCP_EDEVICE (-2)
+#define LDPCP_ENOMEM (-3)
+#define LDPCP_ERANGE (-4)   /* a trajectory left its barrier window or replay stream */
+
+/* Parameters of one reference run (the option table of
+ * langevinDynamicsSingleBarrier.cpp:65-108 / ...DoubleBarrier.cpp:80-85 as
+ * copied into the integrator by LangevinEquationRunnerMPI::Run,
+ * langevinDynamicsSingleBarrier.h:3249-3301). */
+typedef struct ldpcp_run_params {
+    int32_t model;                 /* 0 single barrier, 1 double barrier */
+    int32_t overdamped;
+    double x0, v0, gamma, kb, temperature, mass;
+    int32_t randomize_v, randomize_state, same_initial_barrier;
+    double periodic_length;
+    double height[2][2];           /* [barrier][state] */
+    double half_width[2];
+    double position[2];
+    double timestep;
+    uint64_t nsteps;               /* NUMBEROFSTEPS */
+    uint64_t save_freq;            /* FREQSAVE */
+    double tau;
+    double initial_state;
+    int32_t equal_probability;
+    double eq_constant;
+    int32_t b_waiting, b_fpt, b_tpt, b_fptcg;
+    int32_t n_cg;
+    int32_t cg_size[LDPCP_MAX_CG];
+    int32_t save_all;              /* WRITEX: keep v, f, pe, ke, state, barrier frames */
+    uint64_t seed;                 /* Philox key (free-running noise) */
+    int32_t barrier_window;        /* barrier entries kept per trajectory and barrier array */
+} ldpcp_run_params;
+
+/* Per-trajectory results: the values the runner reads through
+ * GetFirstPassageTimeAverage{,2}, GetTransitionPathTimeAverage{,2},
+ * GetFirstPassageTimeCGAverage2, GetStateCount, GetV0Instance,
+ * GetBarrierState0Instance and GetBarrierTick
+ * (langevinDynamicsSingleBarrier.h:2620-2769), kept as running sums and
+ * counts so the device never stores event lists. */
+typedef struct ldpcp_lane {
+    double x, v, f, e, r_prev, state, barrier;
+    double state_count[2];
+    double well, periodic, xp, local_barrier;
+    double v0_instance, state0_instance;
+    double fpt_sum, fpt_last_tick; uint64_t fpt_n; int32_t fpt_to_update;
+    double fpt2_sum, fpt2_last_tick; uint64_t fpt2_n; int32_t fpt2_to_update; int32_t fpt2_idx[2];
+    double tpt_sum, tpt_last_tick; uint64_t tpt_n; int32_t tpt_to_update;
+    double tpt2_sum, tpt2_last_tick; uint64_t tpt2_n; int32_t tpt2_to_update; int32_t tpt2_initial_well;
+    double cg_sum[LDPCP_MAX_CG], cg_last_tick[LDPCP_MAX_CG];
+    uint64_t cg_n[LDPCP_MAX_CG]; int32_t cg_to_update[LDPCP_MAX_CG]; int32_t cg_initial_well[LDPCP_MAX_CG];
+    double wt_tick_prev; int32_t wt_passed;
+    uint64_t wt_ticks; double wt_last; double wt_sum, wt_sum2, wt_max; uint64_t wt_n;
+    uint64_t absorbed_step, tpt_start_step, steps_done;
+    uint64_t n_gauss, n_flip, n_catch;
+    int32_t error;
+} ldpcp_lane;
+
+typedef struct ldpcp_run ldpcp_run;
+
+/* Replaces `new LangevinEquationSIMDMultiple` / `Overdamped...` + the
+ * parameter copy + Initialize(seed) (langevinDynamicsSingleBarrier.h:3231-3306,
+ * :1276-1291).  One handle holds n_traj trajectories ("lanes"); the Philox
+ * counter of trajectory i is keyed by traj_offset + i, so results do not
+ * depend on how trajectories are split over handles. */
+int ldpcp_run_create(const ldpcp_run_params* params, uint32_t n_traj, uint32_t traj_offset, ldpcp_run** out);
+
+/* Replay mode (no reference counterpart: the reference's own MKL banks,
+ * RANDOMNUMBER.cpp:157-340, are replaced by role-tagged per-trajectory
+ * streams).  gauss and flip are [k][n_traj]; catchup is [n_traj][catch_cap];
+ * binit is [barrier array][n_traj][bn] holding each barrier's state at
+ * creation for ids bmin .. bmin+bn-1 (-1 = never created). */
+int ldpcp_run_set_replay(ldpcp_run* h, const double* gauss, uint64_t n_gauss,
+                         const double* flip, uint64_t n_flip,
+                         const double* catchup, uint64_t catch_cap,
+                         const int8_t* binit, int32_t bmin, int32_t bn);
+
+/* Replaces Integrate(iTrial, bTimer) (langevinDynamicsSingleBarrier.h:1386-1519):
+ * step 0, then NSTEPS steps (trailing NSTEPS mod FREQSAVE steps dropped when
+ * FREQSAVE >= 10), frames every FREQSAVE steps, statistics every step. */
+int ldpcp_run_integrate(ldpcp_run* h, uint64_t steps_per_launch);
+
+/* Per-trajectory results for trajectories [begin, begin+n). */
+int ldpcp_run_lanes(ldpcp_run* h, uint32_t begin, uint32_t n, ldpcp_lane* out);
+
+/* Replaces GetNumberOfFrameX and GetX/GetV/GetF/GetPE/GetKE/GetState/GetBarrier
+ * (langevinDynamicsSingleBarrier.h:2629-2701).  field: 0 x, 1 v, 2 f, 3 pe,
+ * 4 ke, 5 state, 6 barrier.  out is [n][n_frames]. */
+uint64_t ldpcp_run_n_frames(const ldpcp_run* h);
+int ldpcp_run_frames(ldpcp_run* h, int32_t field, uint32_t begin, uint32_t n, double* out);
+
+/* Replaces MeanSquareDisplacementBruteForce over each lane's saved x frames
+ * (langevinDynamicsSingleBarrier.h:24-45, called at :3454).  out is
+ * [n_traj][nlags]: the per-lane, time-origin-averaged MSD at each lag (in
+ * frames), in the reference's summation order. */
+int ldpcp_run_msd(ldpcp_run* h, const uint64_t* lags, uint32_t nlags, double* out);
+
+/* Replaces SAFE_DELETE(leOMP) (langevinDynamicsSingleBarrier.h:3972). */
+int ldpcp_run_destroy(ldpcp_run* h);
+
+/* ------------------------------------------------------------------------- */
+/* Sweep engine: absorbing first-passage runs over ragged parameter sets.       */
+/* No reference counterpart (the reference runs one parameter file per MPI     */
+/* job); it is the batched form of the same integrator, stopped at the first    */
+/* barrier-top crossing, with FPT/TPT/MSD statistics reduced on the device.     */
+/* ------------------------------------------------------------------------- */
+#define LDPCP_HIST_BINS 256
+#define LDPCP_MSD_LAGS 64
+
+typedef struct ldpcp_sweep_set {
+    int32_t model, overdamped;
+    double x0, v0, gamma, kb, temperature, mass;
+    int32_t randomize_v, randomize_state;
+    double initial_state;
+    double periodic_length;
+    double height[2][2];
+    double half_width[2];
+    double position[2];
+    double timestep, tau;
+    int32_t equal_probability;
+    double eq_constant;
+    uint64_t n_traj;
+    uint64_t max_steps;            /* step cap per trajectory */
+    uint64_t seed;
+} ldpcp_sweep_set;
+
+typedef struct ldpcp_set_stats {
+    uint64_t n_traj, n_absorbed, n_capped;
+    double fpt_sum, fpt_sum2;      /* first-passage times of absorbed trajectories */
+    double tpt_sum, tpt_sum2;      /* transition-path times of absorbed trajectories */
+    uint64_t traj_steps;           /* trajectory-steps integrated */
+    double hist_lo, hist_hi;       /* log10 edges of the FPT histogram, in time units */
+    uint64_t hist[LDPCP_HIST_BINS];
+    uint64_t msd_step[LDPCP_MSD_LAGS];
+    double msd_sum[LDPCP_MSD_LAGS];
+    uint64_t msd_n[LDPCP_MSD_LAGS];
+} ldpcp_set_stats;
+
+typedef struct ldpcp_sweep ldpcp_sweep;
+
+/* precision: 64 = plain FP64 stepping (the reference arithmetic), 32 = FP32
+ * stepping with a compensated (cell, offset) position and an integer clock. */
+int ldpcp_sweep_create(const ldpcp_sweep_set* sets, int32_t nsets, int32_t precision, ldpcp_sweep** out);
+/* One launch over every tile that still has live trajectories; returns the
+ * number of live trajectories left in *live. */
+int ldpcp_sweep_launch(ldpcp_sweep* h, uint32_t steps_per_launch, uint64_t* live);
+/* Launches until every trajectory is absorbed or capped. */
+int ldpcp_sweep_run(ldpcp_sweep* h, uint32_t steps_per_launch);
+int ldpcp_sweep_stats(ldpcp_sweep* h, int32_t set, ldpcp_set_stats* out);
+/* Per-trajectory absorption step (0 = capped) and last well-centre crossing. */
+int ldpcp_sweep_fpt(ldpcp_sweep* h, int32_t set, uint64_t* absorbed_step, uint64_t* tpt_start);
+/* Mean duration (ms) of the stepping kernel over the launches so far, timed
+ * with HIP events on the handle's stream, and the launch count. */
+int ldpcp_sweep_kernel_time(ldpcp_sweep* h, double* ms_total, uint64_t* launches);
+int ldpcp_sweep_destroy(ldpcp_sweep* h);
+
+/* Library-wide. */
+const char* ldpcp_last_error(void);
+int ldpcp_device_info(char* name, int32_t name_len, int32_t* n_cu, int64_t* hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

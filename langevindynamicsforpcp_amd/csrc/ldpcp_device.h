@@ -1,0 +1,86 @@
+// ldpcp_device.h -- noise primitives shared by the run-mode and sweep kernels.
+//
+// Counter-based Philox4x32-10 (Salmon et al., "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11) replaces the reference's MKL MT19937 banks
+// (RANDOMNUMBER.cpp:157-340).  Every draw is a pure function of
+// (seed, trajectory id, role, index), so a trajectory's noise does not depend
+// on which thread, launch or GPU integrates it.
+//
+// Counter layout {index_lo, index_hi, trajectory, role}, key {seed_lo, seed_hi}:
+//   role 0: step noise, one call per pair of steps (steps 2q+1, 2q+2):
+//           words 0,1 -> Box-Muller pair (cos -> odd step, sin -> even step),
+//           words 2,3 -> per-step flip uniforms (odd, even step)
+//   role 1: initial draws, index 0: words 0,1 -> Box-Muller (cos -> V0, sin -> R0),
+//           word 2 -> initial barrier state uniform
+//   role 2: catch-up uniform of the trajectory's n-th barrier hand-over (word 0)
+//   role 3 + 256*array: state of barrier `id` when first touched (word 0)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ldpcp {
+
+struct Philox4 {
+    uint32_t w[4];
+};
+
+__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(a, b);
+#else
+    return (uint32_t)(((uint64_t)a * b) >> 32);
+#endif
+}
+
+__host__ __device__ __forceinline__ Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                         uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = mulhi32(0xD2511F53u, c0);
+        const uint32_t lo0 = 0xD2511F53u * c0;
+        const uint32_t hi1 = mulhi32(0xCD9E8D57u, c2);
+        const uint32_t lo1 = 0xCD9E8D57u * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    Philox4 out;
+    out.w[0] = c0;
+    out.w[1] = c1;
+    out.w[2] = c2;
+    out.w[3] = c3;
+    return out;
+}
+
+// (w + 1/2) / 2^32: exact in FP64, never 0 or 1.
+__host__ __device__ __forceinline__ double unit_f64(uint32_t w)
+{
+    return ((double)w + 0.5) * 2.3283064365386963e-10;
+}
+
+__host__ __device__ __forceinline__ void box_muller_f64(uint32_t a, uint32_t b, double& g0, double& g1)
+{
+    const double u1 = unit_f64(a), u2 = unit_f64(b);
+    const double r = sqrt(-2.0 * log(u1));
+    const double th = 6.283185307179586 * u2;
+    g0 = r * cos(th);
+    g1 = r * sin(th);
+}
+
+// Integer form of "u < p" for u = (w + 1/2) / 2^32: w < ceil(p 2^32 - 1/2).
+__host__ __device__ __forceinline__ uint64_t flip_threshold(double p)
+{
+    double P = ceil(p * 4294967296.0 - 0.5);
+    if (P < 0.0) P = 0.0;
+    if (P > 4294967296.0) P = 4294967296.0;
+    return (uint64_t)P;
+}
+
+}  // namespace ldpcp

@@ -1,0 +1,999 @@
+// ldpcp_run.hip -- reference-semantics integrator on the GPU (the drop-in path).
+//
+// One thread owns one trajectory ("lane" of the reference's SIMD vector).
+// The step arithmetic follows the reference operation by operation in FP64
+// (this file is compiled with -ffp-contract=off; the single fused
+// multiply-add the reference requests, _mmSIMD_fmadd_pd in the overdamped
+// position update, langevinDynamicsSingleBarrier.h:2868, is an explicit
+// fma()).  The hot state (position, velocity, force, barrier state, indices)
+// lives in VGPRs for a whole launch; statistics and the per-trajectory barrier
+// table live in HBM and are touched only on the rare crossing events.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "ldpcp_common.hpp"
+#include "ldpcp_device.h"
+
+namespace ldpcp {
+
+namespace {
+
+constexpr int kFields = 7;  // x, v, f, pe, ke, state, barrier
+constexpr uint64_t kValid = 1ull << 63;
+constexpr uint64_t kState = 1ull << 62;
+constexpr uint64_t kStepMask = kState - 1;
+
+struct RunConst {
+    int32_t model, overdamped, tau_zero, equal_prob, same_init, centre;
+    int32_t b_wait, b_fpt, b_tpt, b_cg, n_cg, save_all, randomize_v, randomize_state, replay, wb;
+    int32_t cg_size[LDPCP_MAX_CG];
+    double dt, minv, mhalf, halfdt, expmgdt, var, dtgm, L, Linv, c_lap, p_flip, p_asym0, p_asym1, K, p0,
+        p_create;
+    double H[2], S[2], w;
+    double Hd[2][2], xb[2], lo[2], hi[2], xc;
+    double slope[2][2][2], icept[2][2][2];
+    double vsigma, x0, v0, initial_state;
+    uint64_t save_freq, nsteps_eff, frames_cap;
+    uint32_t key0, key1, n_traj, traj_offset;
+    uint64_t n_gauss, n_flip, catch_cap, init_gauss, init_flip;
+    int32_t bmin, bn;
+};
+
+struct RunPtrs {
+    double *x, *v, *F, *E, *R, *barrier, *state, *well, *periodic, *xp, *local;
+    uint64_t *cnt0, *cnt1;
+    ldpcp_lane* lanes;
+    uint64_t* win;       // [traj][2][wb]
+    int32_t* wbase;      // [traj]
+    double* frames;      // [field][frame][traj]
+    const double* g;     // replay gaussians [k][traj]
+    const double* f;     // replay flip uniforms [k][traj]
+    const double* c;     // replay catch-up uniforms [traj][cap]
+    const int8_t* binit; // replay barrier states [array][traj][bn]
+};
+
+// Constants of IntegrateInitializeVariables (langevinDynamicsSingleBarrier.h:1520-1618,
+// overdamped :2812-2824; langevinDynamicsDoubleBarrier.h:1733-1937, GetXc :1722-1732).
+void make_const(const ldpcp_run_params& p, RunConst& c)
+{
+    memset(&c, 0, sizeof(c));
+    c.model = p.model;
+    c.overdamped = p.overdamped;
+    c.tau_zero = (p.tau == 0.0);
+    c.equal_prob = p.equal_probability;
+    c.same_init = p.same_initial_barrier;
+    c.b_wait = p.b_waiting;
+    c.b_fpt = p.b_fpt;
+    c.b_tpt = p.b_tpt;
+    c.b_cg = p.b_fptcg;
+    c.n_cg = p.n_cg;
+    for (int j = 0; j < p.n_cg && j < LDPCP_MAX_CG; ++j) c.cg_size[j] = p.cg_size[j];
+    c.save_all = p.save_all;
+    c.randomize_v = p.randomize_v;
+    c.randomize_state = p.randomize_state;
+    c.wb = p.barrier_window;
+    c.dt = p.timestep;
+    c.minv = 1.0 / p.mass;
+    c.mhalf = p.mass * 0.5;
+    c.halfdt = p.timestep * 0.5;
+    c.expmgdt = exp(-p.gamma * p.timestep);
+    c.var = sqrt(p.kb * p.temperature / p.mass * (1.0 - exp(-2.0 * p.gamma * p.timestep)));
+    c.L = p.periodic_length;
+    c.K = p.eq_constant;
+    if (p.equal_probability) {
+        c.c_lap = -2.0 * p.timestep / p.tau;
+        c.p_flip = c.tau_zero ? 0.0 : 0.5 * (1.0 - exp(-2.0 * p.timestep / p.tau));
+        c.p0 = 0.5;
+    } else {
+        c.c_lap = -(p.eq_constant + 1.0) * p.timestep / p.tau;
+        if (!c.tau_zero) {
+            c.p_asym0 = p.eq_constant / (1.0 + p.eq_constant) * (1.0 - exp(c.c_lap));
+            c.p_asym1 = 1.0 / (1.0 + p.eq_constant) * (1.0 - exp(c.c_lap));
+        }
+        c.p0 = 1.0 / (1.0 + p.eq_constant);
+    }
+    c.p_create = 1.0 / (1.0 + p.eq_constant);
+    c.Linv = 1.0 / c.L;
+    if (p.model == 0) {
+        c.w = p.half_width[0];
+        for (int s = 0; s < 2; ++s) {
+            c.H[s] = p.height[0][s];
+            c.S[s] = p.height[0][s] / p.half_width[0];
+        }
+    } else {
+        for (int b = 0; b < 2; ++b) {
+            for (int s = 0; s < 2; ++s) c.Hd[b][s] = p.height[b][s];
+            c.xb[b] = p.position[b];
+        }
+        const double x1b = p.position[0], w1 = p.half_width[0], x2b = p.position[1], w2 = p.half_width[1];
+        const double y1 = p.height[0][0], y2 = p.height[1][0];
+        const double x1h = x1b + w1, x2l = x2b - w2;
+        const double xc = fma(x1h * y1, x2b - x2l, x2l * y2 * (x1h - x1b)) / fma(y1, x2b - x2l, y2 * (x1h - x1b));
+        const double yc = (x1h - xc) / (x1h - x1b) * y1;
+        c.xc = xc;
+        c.lo[0] = x1b - w1;
+        c.hi[0] = x1b + w1;
+        c.lo[1] = x2b - w2;
+        c.hi[1] = x2b + w2;
+        const double xb_xl[2] = {c.xb[0] - c.lo[0], c.xb[1] - c.lo[1]};
+        const double xh_xb[2] = {c.hi[0] - c.xb[0], c.hi[1] - c.xb[1]};
+        for (int s = 0; s < 2; ++s) {
+            const double H0 = c.Hd[0][s], H1 = c.Hd[1][s];
+            c.slope[0][s][0] = H0 / xb_xl[0];
+            c.slope[0][s][1] = (yc - H0) / (xc - c.xb[0]);
+            c.slope[1][s][0] = (H1 - yc) / (c.xb[1] - xc);
+            c.slope[1][s][1] = 0.0 - H1 / xh_xb[1];
+            c.icept[0][s][0] = 0.0 - H0 * (c.lo[0] / xb_xl[0]);
+            c.icept[0][s][1] = (xc * H0 - yc * c.xb[0]) / (xc - c.xb[0]);
+            c.icept[1][s][0] = (yc * c.xb[1] - xc * H1) / (c.xb[1] - xc);
+            c.icept[1][s][1] = H1 * (c.hi[1] / xh_xb[1]);
+        }
+    }
+    if (p.overdamped) {
+        const double mgi = 1.0 / (p.mass * p.gamma);
+        c.dtgm = p.timestep * mgi;
+        c.var = sqrt(p.kb * p.temperature * mgi * p.timestep * 0.5);
+    }
+    c.vsigma = sqrt(p.kb * p.temperature / p.mass);
+    c.x0 = p.x0;
+    c.v0 = p.v0;
+    c.initial_state = p.initial_state;
+    const double m = fmod(fmod(p.x0, p.periodic_length) + p.periodic_length, p.periodic_length);
+    c.centre = (m == p.periodic_length * 0.5);
+    c.save_freq = p.save_freq;
+    c.nsteps_eff = p.save_freq >= 10 ? (p.nsteps / p.save_freq) * p.save_freq : p.nsteps;
+    c.frames_cap = c.nsteps_eff / p.save_freq + 1;
+    c.key0 = (uint32_t)p.seed;
+    c.key1 = (uint32_t)(p.seed >> 32);
+    c.init_gauss = (uint64_t)(p.randomize_v ? 1 : 0) + (uint64_t)(p.overdamped ? 1 : 0);
+    c.init_flip = p.randomize_state ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Per-lane step pieces, each restating one reference method.
+// ---------------------------------------------------------------------------
+struct Lane {
+    double x, v, F, E, R, barrier, state;
+    double height, slope_pos;                       // single barrier
+    double dh[2], ds[2][2], di[2][2];               // double barrier
+    double well, periodic, well_prev, periodic_prev, xp, local, local_prev;
+    bool ent_well, ent_period, ent_local;
+    uint64_t cnt0, cnt1;
+};
+
+// UpdateBarrierStateEtaParamter (Single.h:2154-2165; Double.h:2737-2757)
+__device__ __forceinline__ void lane_param(Lane& s, const RunConst& c, bool count = true)
+{
+    const bool one = (s.state == 1.0);
+    if (c.model == 0) {
+        s.height = one ? c.H[1] : c.H[0];
+        s.slope_pos = one ? c.S[1] : c.S[0];
+    } else {
+        const int st = one ? 1 : 0;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            s.dh[b] = c.Hd[b][st];
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                s.ds[b][side] = c.slope[b][st][side];
+                s.di[b][side] = c.icept[b][st][side];
+            }
+        }
+    }
+    if (!count) return;
+    if (one) s.cnt1 += 1;
+    else s.cnt0 += 1;
+}
+
+// EnergyAndForce (Single.h:1292-1324; Double.h:1464-1522)
+__device__ __forceinline__ void lane_energy_force(Lane& s, const RunConst& c)
+{
+    if (c.model == 0) {
+        const double r = s.x * c.Linv;
+        const double pidx = rint(r);
+        const double xp = s.x - pidx * c.L;
+        const double slope_neg = 0.0 - s.slope_pos;
+        const bool neg = (xp < 0.0) && (xp > -c.w);
+        const bool pos = (xp < c.w) && (xp > 0.0);
+        const double vneg = s.height - xp * slope_neg;
+        const double vpos = s.height - xp * s.slope_pos;
+        double E = neg ? vneg : 0.0;
+        E = pos ? vpos : E;
+        double F = neg ? slope_neg : 0.0;
+        F = pos ? s.slope_pos : F;
+        s.E = E;
+        s.F = F;
+        s.barrier = s.height;
+    } else {
+        const double xp = s.xp;
+        const bool n0 = (xp < c.xb[0]) && (xp > c.lo[0]);
+        const bool p0 = (xp < c.xc) && (xp > c.xb[0]);
+        const bool n1 = (xp < c.xb[1]) && (xp > c.xc);
+        const bool p1 = (xp < c.hi[1]) && (xp > c.xb[1]);
+        const double v00 = xp * s.ds[0][0] + s.di[0][0];
+        const double v01 = xp * s.ds[0][1] + s.di[0][1];
+        const double v10 = xp * s.ds[1][0] + s.di[1][0];
+        const double v11 = xp * s.ds[1][1] + s.di[1][1];
+        double E = n0 ? v00 : 0.0;
+        E = p0 ? v01 : E;
+        E = n1 ? v10 : E;
+        E = p1 ? v11 : E;
+        double F = n0 ? s.ds[0][0] : 0.0;
+        F = p0 ? s.ds[0][1] : F;
+        F = n1 ? s.ds[1][0] : F;
+        F = p1 ? s.ds[1][1] : F;
+        F = 0.0 - F;
+        double B = n0 ? s.dh[0] : 0.0;
+        B = p0 ? s.dh[0] : B;
+        B = n1 ? s.dh[1] : B;
+        B = p1 ? s.dh[1] : B;
+        s.E = E;
+        s.F = F;
+        s.barrier = B;
+    }
+}
+
+// InitializePeriodicIndices / UpdatePeriodicIndices (Single.h:1838-1884; Double.h:2150-2235)
+__device__ __forceinline__ void lane_indices(Lane& s, const RunConst& c, bool init)
+{
+    const double r = s.x * c.Linv;
+    const double well = floor(r), periodic = rint(r);
+    if (init) {
+        s.well = s.well_prev = well;
+        s.periodic = s.periodic_prev = periodic;
+        s.ent_well = s.ent_period = false;
+    } else {
+        s.well_prev = s.well;
+        s.periodic_prev = s.periodic;
+        s.well = well;
+        s.periodic = periodic;
+        s.ent_well = (s.well_prev != s.well);
+        s.ent_period = (s.periodic_prev != s.periodic);
+    }
+    if (c.model == 1) {
+        s.xp = s.x - s.periodic * c.L;
+        const double local = (s.xp < c.xc) ? 0.0 : 1.0;
+        if (init) {
+            s.local = s.local_prev = local;
+            s.ent_local = false;
+        } else {
+            s.local_prev = s.local;
+            s.local = local;
+            s.ent_local = (s.local_prev != s.local);
+        }
+    } else {
+        s.ent_local = false;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Noise
+// ---------------------------------------------------------------------------
+struct StepNoise {
+    uint64_t q;  // cached Philox pair index (+1), 0 = none
+    double g1;
+    uint32_t w3;
+};
+
+__device__ __forceinline__ void step_noise(const RunConst& c, const RunPtrs& P, uint32_t t, uint64_t iStep,
+                                           StepNoise& sn, double& g, double& u)
+{
+    if (c.replay) {
+        const uint64_t k = iStep - 1;
+        g = P.g[(c.init_gauss + k) * c.n_traj + t];
+        u = c.tau_zero ? 1.0 : P.f[(c.init_flip + k) * c.n_traj + t];
+        return;
+    }
+    const uint64_t q = (iStep - 1) >> 1;
+    if ((iStep - 1) & 1) {
+        if (sn.q != q + 1) {
+            const Philox4 w = philox4x32_10((uint32_t)q, (uint32_t)(q >> 32), c.traj_offset + t, 0u, c.key0, c.key1);
+            double g0;
+            box_muller_f64(w.w[0], w.w[1], g0, sn.g1);
+            sn.w3 = w.w[3];
+        }
+        g = sn.g1;
+        u = unit_f64(sn.w3);
+    } else {
+        const Philox4 w = philox4x32_10((uint32_t)q, (uint32_t)(q >> 32), c.traj_offset + t, 0u, c.key0, c.key1);
+        box_muller_f64(w.w[0], w.w[1], g, sn.g1);
+        sn.w3 = w.w[3];
+        sn.q = q + 1;
+        u = unit_f64(w.w[2]);
+    }
+}
+
+__device__ __forceinline__ int barrier_init_state(const RunConst& c, const RunPtrs& P, uint32_t t, int array, int id,
+                                                  int& err)
+{
+    if (c.same_init) return 0;  // UserData::m_bInitialValue of a fresh run (Single.h:1378-1380)
+    if (c.replay) {
+        const int k = id - c.bmin;
+        if (k < 0 || k >= c.bn) {
+            err = 2;
+            return 0;
+        }
+        const int8_t v = P.binit[((uint64_t)array * c.n_traj + t) * c.bn + k];
+        if (v < 0) {
+            err = 2;
+            return 0;
+        }
+        return v;
+    }
+    const Philox4 w = philox4x32_10((uint32_t)id, 0u, c.traj_offset + t, 3u + 256u * (uint32_t)array, c.key0, c.key1);
+    return unit_f64(w.w[0]) < c.p_create ? 0 : 1;
+}
+
+// Barrier table entry for (array, id): valid bit, state bit, 62-bit time step.
+__device__ __forceinline__ uint64_t* win_entry(const RunConst& c, const RunPtrs& P, uint32_t t, int array, int id,
+                                               int& err)
+{
+    const int k = id - P.wbase[t];
+    if (k < 0 || k >= c.wb) {
+        err = 3;
+        return nullptr;
+    }
+    return P.win + ((uint64_t)t * 2 + array) * c.wb + k;
+}
+
+__device__ __forceinline__ uint64_t win_load(const RunConst& c, const RunPtrs& P, uint32_t t, int array, int id,
+                                             uint64_t* e, int& err)
+{
+    uint64_t v = *e;
+    if (!(v & kValid)) {
+        v = kValid | (barrier_init_state(c, P, t, array, id, err) ? kState : 0ull);  // created at step 0
+        *e = v;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Statistics state machines (Single.h:2166-2592; Double.h:2758-3209)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void arm_indices(int idx[2], const Lane& s)
+{
+    const int pidx = (int)s.periodic;
+    if (s.well == s.periodic) {
+        idx[0] = pidx;
+        idx[1] = pidx + 1;
+    } else {
+        idx[0] = pidx - 1;
+        idx[1] = pidx;
+    }
+}
+
+__device__ void stats_init(const RunConst& c, ldpcp_lane* o, const Lane& s)
+{
+    if (c.b_wait) {
+        o->wt_tick_prev = 0.0;
+        o->wt_passed = 0;
+    }
+    if (c.b_fpt) {
+        o->fpt_last_tick = 0.0;
+        o->fpt_to_update = c.centre ? 0 : 1;
+        o->fpt2_last_tick = 0.0;
+        if (c.centre) {
+            o->fpt2_to_update = 0;
+            int idx[2];
+            arm_indices(idx, s);
+            o->fpt2_idx[0] = idx[0];
+            o->fpt2_idx[1] = idx[1];
+        } else {
+            o->fpt2_to_update = 1;
+        }
+    }
+    if (c.b_tpt) {
+        o->tpt_last_tick = 0.0;
+        o->tpt_to_update = c.centre ? 0 : 1;
+        o->tpt2_last_tick = 0.0;
+        o->tpt2_to_update = c.centre ? 0 : 1;
+        o->tpt2_initial_well = (int)s.well;
+    }
+    if (c.b_cg) {
+        for (int j = 0; j < c.n_cg; ++j) {
+            o->cg_last_tick[j] = 0.0;
+            o->cg_initial_well[j] = (int)s.well;
+            o->cg_to_update[j] = !(c.centre && ((int)rint(s.well) % c.cg_size[j] == 0));
+        }
+    }
+}
+
+// Event duration `m_TimeStep0 * iStep - lastTick` (Single.h:2398, 2453, 2509,
+// 2525, 2576): a scalar statement the reference's compiler contracts into one
+// fused multiply-add; the recorded event lists of the quick test match only
+// this form.
+__device__ __forceinline__ double ev(const RunConst& c, uint64_t iStep, double last)
+{
+    return fma(c.dt, (double)iStep, -last);
+}
+
+__device__ void stats_event(const RunConst& c, ldpcp_lane* o, const Lane& s, uint64_t iStep)
+{
+    const double t = c.dt * (double)iStep;
+    const bool single = (c.model == 0);
+    if (c.b_wait && single) {
+        if (s.ent_well) {
+            o->wt_tick_prev = (double)iStep;
+            o->wt_passed = 1;
+        } else if (s.ent_period && o->wt_passed) {
+            const double tick = o->wt_tick_prev;
+            if (o->wt_ticks > 0) {
+                const double w = (tick - o->wt_last) * c.dt;
+                o->wt_sum += w;
+                o->wt_sum2 = fma(w, w, o->wt_sum2);
+                if (o->wt_n == 0 || w > o->wt_max) o->wt_max = w;
+                o->wt_n++;
+            }
+            o->wt_last = tick;
+            o->wt_ticks++;
+            o->wt_passed = 0;
+        }
+    }
+    if (c.b_fpt) {
+        if (single) {
+            if (o->fpt_to_update && s.ent_period) {
+                o->fpt_to_update = 0;
+                o->fpt_last_tick = t;
+            }
+            if (!o->fpt_to_update && s.ent_well) {
+                o->fpt_to_update = 1;
+                o->fpt_sum += ev(c, iStep, o->fpt_last_tick);
+                o->fpt_n++;
+            }
+        }
+        if (o->fpt2_to_update && s.ent_period) {
+            o->fpt2_to_update = 0;
+            o->fpt2_last_tick = t;
+            int idx[2];
+            arm_indices(idx, s);
+            o->fpt2_idx[0] = idx[0];
+            o->fpt2_idx[1] = idx[1];
+        }
+        if (s.ent_period && !o->fpt2_to_update && ((double)o->fpt2_idx[0] != s.periodic) &&
+            ((double)o->fpt2_idx[1] != s.periodic)) {
+            o->fpt2_sum += ev(c, iStep, o->fpt2_last_tick);
+            o->fpt2_n++;
+            o->fpt2_last_tick = t;
+            int idx[2];
+            arm_indices(idx, s);
+            o->fpt2_idx[0] = idx[0];
+            o->fpt2_idx[1] = idx[1];
+        }
+    }
+    if (c.b_tpt && single) {
+        if (s.ent_period) {
+            o->tpt_to_update = 0;
+            o->tpt_last_tick = t;
+        }
+        if (!o->tpt_to_update && s.ent_well) {
+            o->tpt_to_update = 1;
+            o->tpt_sum += ev(c, iStep, o->tpt_last_tick);
+            o->tpt_n++;
+        }
+        if (s.ent_period) {
+            if (o->tpt2_to_update) {
+                o->tpt2_to_update = 0;
+            } else if (s.well != (double)o->tpt2_initial_well) {
+                o->tpt2_sum += ev(c, iStep, o->tpt2_last_tick);
+                o->tpt2_n++;
+            }
+            o->tpt2_last_tick = t;
+            o->tpt2_initial_well = (int)s.well;
+        }
+    }
+    if (c.b_cg && s.ent_period) {
+        for (int j = 0; j < c.n_cg; ++j) {
+            if ((int)rint(s.well) % c.cg_size[j] != 0) continue;
+            if (o->cg_to_update[j]) {
+                o->cg_to_update[j] = 0;
+                o->cg_last_tick[j] = t;
+                o->cg_initial_well[j] = (int)s.well;
+            } else if (s.well != (double)o->cg_initial_well[j]) {
+                o->cg_sum[j] += ev(c, iStep, o->cg_last_tick[j]);
+                o->cg_n[j]++;
+                o->cg_last_tick[j] = t;
+                o->cg_initial_well[j] = (int)s.well;
+            }
+        }
+    }
+}
+
+// Record (Single.h:1707-1765); frames are [field][frame][traj]
+__device__ __forceinline__ void record(const RunConst& c, const RunPtrs& P, uint32_t t, const Lane& s, uint64_t frame)
+{
+    if (frame >= c.frames_cap) return;
+    const uint64_t plane = c.frames_cap * c.n_traj;
+    const uint64_t o = frame * c.n_traj + t;
+    P.frames[o] = s.x;
+    if (c.save_all) {
+        P.frames[plane * 1 + o] = c.overdamped ? 0.0 : s.v;
+        P.frames[plane * 2 + o] = s.F;
+        P.frames[plane * 3 + o] = s.E;
+        P.frames[plane * 4 + o] = c.mhalf * (s.v * s.v);
+        P.frames[plane * 5 + o] = s.state;
+        P.frames[plane * 6 + o] = s.barrier;
+    }
+}
+
+__device__ __forceinline__ void load_lane(Lane& s, const RunConst& c, const RunPtrs& P, uint32_t t)
+{
+    s.x = P.x[t];
+    s.v = P.v[t];
+    s.F = P.F[t];
+    s.E = P.E[t];
+    s.R = P.R[t];
+    s.barrier = P.barrier[t];
+    s.state = P.state[t];
+    s.well = P.well[t];
+    s.periodic = P.periodic[t];
+    s.xp = P.xp[t];
+    s.local = P.local[t];
+    s.cnt0 = P.cnt0[t];
+    s.cnt1 = P.cnt1[t];
+    s.ent_well = s.ent_period = s.ent_local = false;
+    lane_param(s, c, false);  // rebuild heights and slopes from the stored state
+}
+
+__device__ __forceinline__ void store_lane(const Lane& s, const RunPtrs& P, uint32_t t)
+{
+    P.x[t] = s.x;
+    P.v[t] = s.v;
+    P.F[t] = s.F;
+    P.E[t] = s.E;
+    P.R[t] = s.R;
+    P.barrier[t] = s.barrier;
+    P.state[t] = s.state;
+    P.well[t] = s.well;
+    P.periodic[t] = s.periodic;
+    P.xp[t] = s.xp;
+    P.local[t] = s.local;
+    P.cnt0[t] = s.cnt0;
+    P.cnt1[t] = s.cnt1;
+}
+
+// IntegrateStep0 (Single.h:1619-1652, overdamped :2825-2832)
+__global__ void __launch_bounds__(256) run_init_kernel(RunConst c, RunPtrs P)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.n_traj) return;
+    ldpcp_lane* o = P.lanes + t;
+    int err = 0;
+    Lane s;
+    s.v = s.F = s.E = s.R = s.barrier = 0.0;
+    s.xp = s.local = s.local_prev = 0.0;
+    s.x = c.x0;
+    lane_indices(s, c, true);
+    Philox4 w0;
+    if (!c.replay) w0 = philox4x32_10(0u, 0u, c.traj_offset + t, 1u, c.key0, c.key1);
+    uint64_t ng = 0, nf = 0;
+    if (c.randomize_state) {
+        const double u = c.replay ? P.f[t] : unit_f64(w0.w[2]);
+        nf++;
+        s.state = (u <= c.p0) ? 0.0 : 1.0;
+    } else {
+        s.state = c.initial_state;
+    }
+    o->state0_instance = s.state;
+    P.wbase[t] = (int)s.periodic - c.wb / 2;
+    {
+        const int a = c.model == 0 ? 0 : (int)s.local;
+        uint64_t* e = win_entry(c, P, t, a, (int)s.periodic, err);
+        if (e) *e = kValid | ((int)s.state ? kState : 0ull);
+    }
+    s.cnt0 = s.cnt1 = 0;
+    lane_param(s, c);
+    double gv = 0.0, gr = 0.0;
+    if (!c.replay) box_muller_f64(w0.w[0], w0.w[1], gv, gr);
+    if (c.randomize_v) {
+        const double g = c.replay ? P.g[ng * c.n_traj + t] : gv;
+        ng++;
+        s.v = c.vsigma * g;
+    } else {
+        s.v = c.v0;
+    }
+    o->v0_instance = s.v;
+    lane_energy_force(s, c);
+    stats_init(c, o, s);
+    if (c.overdamped) {
+        const double g = c.replay ? P.g[ng * c.n_traj + t] : gr;
+        ng++;
+        s.R = c.var * g;
+    }
+    record(c, P, t, s, 0);
+    o->n_gauss = ng;
+    o->n_flip = nf;
+    o->error = err;
+    store_lane(s, P, t);
+}
+
+// Steps (s0, s1] of IntegrateStepEta / IntegrateStep
+// (underdamped Single.h:1654-1706; overdamped :2833-2886; Double.h:1971-2024, 3475-3508)
+__global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, uint64_t s0, uint64_t s1)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.n_traj) return;
+    ldpcp_lane* o = P.lanes + t;
+    if (o->error) return;
+    int err = 0;
+    Lane s;
+    load_lane(s, c, P, t);
+    StepNoise sn;
+    sn.q = 0;
+    sn.g1 = 0.0;
+    sn.w3 = 0;
+    uint64_t n_catch = o->n_catch;
+    uint64_t iStep;
+    for (iStep = s0 + 1; iStep <= s1; ++iStep) {
+        double g, u;
+        step_noise(c, P, t, iStep, sn, g, u);
+        if (c.overdamped) {
+            const double rf = c.var * g;
+            s.x = fma(s.F, c.dtgm, s.x);
+            s.x = s.x + rf;
+            s.x = s.x + s.R;
+            s.R = rf;
+        } else {
+            s.v = s.v + (c.minv * s.F) * c.halfdt;
+            s.x = s.x + s.v * c.halfdt;
+            s.v = c.expmgdt * s.v + c.var * g;
+            s.x = s.x + s.v * c.halfdt;
+        }
+        lane_indices(s, c, false);
+        // per-step flip of the barrier under the particle (Single.h:2010-2015, 2078-2086)
+        if (!c.tau_zero) {
+            if (c.equal_prob) {
+                if (u < c.p_flip) s.state = (s.state == 0.0) ? 1.0 : 0.0;
+            } else {
+                const bool one = (s.state == 1.0);
+                if (u < (one ? c.p_asym1 : c.p_asym0)) s.state = one ? 0.0 : 1.0;
+            }
+        }
+        // hand-over to the barrier just entered (Single.h:2025-2068; Double.h:2466-2510)
+        const bool entering = c.model == 0 ? s.ent_period : s.ent_local;
+        if (entering) {
+            const int ap = c.model == 0 ? 0 : (int)s.local_prev;
+            const int ac = c.model == 0 ? 0 : (int)s.local;
+            uint64_t* ep = win_entry(c, P, t, ap, (int)s.periodic_prev, err);
+            uint64_t* ec = win_entry(c, P, t, ac, (int)s.periodic, err);
+            if (err) break;
+            *ep = kValid | ((int)s.state ? kState : 0ull) | (iStep & kStepMask);
+            const uint64_t ev = win_load(c, P, t, ac, (int)s.periodic, ec, err);
+            const double prev_state = (ev & kState) ? 1.0 : 0.0;
+            if (c.tau_zero) {
+                s.state = prev_state;
+            } else {
+                const double lap = (double)(iStep - (ev & kStepMask));
+                double pc;
+                if (c.equal_prob) pc = 0.5 * (1.0 - exp(c.c_lap * lap));
+                else if (prev_state == 0.0) pc = c.K / (1.0 + c.K) * (1.0 - exp(c.c_lap * lap));
+                else pc = 1.0 / (1.0 + c.K) * (1.0 - exp(c.c_lap * lap));
+                double uc;
+                if (c.replay) {
+                    if (n_catch >= c.catch_cap) {
+                        err = 4;
+                        break;
+                    }
+                    uc = P.c[(uint64_t)t * c.catch_cap + n_catch];
+                } else {
+                    const Philox4 w = philox4x32_10((uint32_t)n_catch, (uint32_t)(n_catch >> 32), c.traj_offset + t, 2u,
+                                                    c.key0, c.key1);
+                    uc = unit_f64(w.w[0]);
+                }
+                n_catch++;
+                s.state = (uc < pc) ? ((prev_state == 1.0) ? 0.0 : 1.0) : prev_state;
+            }
+            if (err) break;
+        }
+        lane_param(s, c);
+        lane_energy_force(s, c);
+        if (!c.overdamped) s.v = s.v + (c.minv * s.F) * c.halfdt;
+        if (s.ent_period || s.ent_well) stats_event(c, o, s, iStep);
+        if (iStep % c.save_freq == 0) record(c, P, t, s, iStep / c.save_freq);
+    }
+    o->n_catch = n_catch;
+    o->n_gauss += (iStep - 1) - s0;
+    if (!c.tau_zero) o->n_flip += (iStep - 1) - s0;
+    o->steps_done = iStep - 1;
+    o->error = err;
+    store_lane(s, P, t);
+}
+
+// MeanSquareDisplacementBruteForce (Single.h:24-45) per (trajectory, lag)
+__global__ void __launch_bounds__(256) msd_kernel(const double* __restrict__ frames, uint64_t n_frames,
+                                                  uint32_t n_traj, const uint64_t* __restrict__ lags,
+                                                  uint32_t nlags, double* __restrict__ out)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t l = blockIdx.y;
+    if (t >= n_traj || l >= nlags) return;
+    const int64_t i = (int64_t)lags[l];
+    const int64_t n = (int64_t)n_frames;
+    double acc = 0.0;
+    if (n - i > 0) {
+        for (int64_t k = 0; k < n - i; ++k) {
+            const double d = frames[(uint64_t)(k + i) * n_traj + t] - frames[(uint64_t)k * n_traj + t];
+            acc = fma(d, d, acc);  // `result += value * value`, contracted by the reference's compiler
+        }
+        acc /= (double)(n - i);
+    }
+    out[(uint64_t)t * nlags + l] = acc;
+}
+
+}  // namespace
+
+}  // namespace ldpcp
+
+struct ldpcp_run {
+    ldpcp_run_params params;
+    ldpcp::RunConst c;
+    ldpcp::RunPtrs P;
+    hipStream_t stream;
+    void* state_block;
+    uint64_t n_frames;
+    bool integrated;
+};
+
+using namespace ldpcp;
+
+static int run_validate(const ldpcp_run_params* p, uint32_t n_traj)
+{
+    if (!p || n_traj == 0) {
+        set_error("ldpcp_run_create: null params or zero trajectories");
+        return LDPCP_EINVAL;
+    }
+    if (p->model != 0 && p->model != 1) {
+        set_error("ldpcp_run_create: model must be 0 (single) or 1 (double), got %d", p->model);
+        return LDPCP_EINVAL;
+    }
+    if (p->save_freq == 0 || p->timestep <= 0.0 || p->periodic_length <= 0.0 || p->mass <= 0.0) {
+        set_error("ldpcp_run_create: save_freq, timestep, periodic_length and mass must be positive");
+        return LDPCP_EINVAL;
+    }
+    if (p->overdamped && p->gamma <= 0.0) {
+        set_error("ldpcp_run_create: the overdamped integrator needs gamma > 0");
+        return LDPCP_EINVAL;
+    }
+    if (p->b_fptcg && (p->n_cg < 0 || p->n_cg > LDPCP_MAX_CG)) {
+        set_error("ldpcp_run_create: n_cg must be in [0, %d]", LDPCP_MAX_CG);
+        return LDPCP_EINVAL;
+    }
+    for (int j = 0; p->b_fptcg && j < p->n_cg; ++j)
+        if (p->cg_size[j] == 0) {
+            set_error("ldpcp_run_create: FPTCG size must be non-zero");
+            return LDPCP_EINVAL;
+        }
+    if (p->barrier_window < 4) {
+        set_error("ldpcp_run_create: barrier_window must be >= 4");
+        return LDPCP_EINVAL;
+    }
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_create(const ldpcp_run_params* p, uint32_t n_traj, uint32_t traj_offset, ldpcp_run** out)
+{
+    if (!out) {
+        set_error("ldpcp_run_create: null output handle");
+        return LDPCP_EINVAL;
+    }
+    *out = nullptr;
+    int rc = run_validate(p, n_traj);
+    if (rc) return rc;
+    ldpcp_run* h = new ldpcp_run();
+    h->params = *p;
+    make_const(*p, h->c);
+    h->c.n_traj = n_traj;
+    h->c.traj_offset = traj_offset;
+    h->n_frames = h->c.frames_cap;
+    const uint64_t n = n_traj;
+    const uint64_t fields = p->save_all ? kFields : 1;
+    const uint64_t bytes_soa = n * (11 * sizeof(double) + 2 * sizeof(uint64_t));
+    const uint64_t bytes_lanes = n * sizeof(ldpcp_lane);
+    const uint64_t bytes_win = n * 2 * (uint64_t)p->barrier_window * sizeof(uint64_t);
+    const uint64_t bytes_wbase = n * sizeof(int32_t);
+    const uint64_t bytes_frames = fields * h->c.frames_cap * n * sizeof(double);
+    const uint64_t total = bytes_soa + bytes_lanes + bytes_win + bytes_wbase + bytes_frames + 1024;
+    hipError_t e = hipMalloc(&h->state_block, total);
+    if (e != hipSuccess) {
+        set_error("ldpcp_run_create: hipMalloc(%llu bytes) failed: %s", (unsigned long long)total, hipGetErrorString(e));
+        delete h;
+        return LDPCP_ENOMEM;
+    }
+    char* b = (char*)h->state_block;
+    double** dptrs[11] = {&h->P.x, &h->P.v, &h->P.F, &h->P.E, &h->P.R, &h->P.barrier, &h->P.state,
+                          &h->P.well, &h->P.periodic, &h->P.xp, &h->P.local};
+    for (int k = 0; k < 11; ++k) {
+        *dptrs[k] = (double*)b;
+        b += n * sizeof(double);
+    }
+    h->P.cnt0 = (uint64_t*)b;
+    b += n * sizeof(uint64_t);
+    h->P.cnt1 = (uint64_t*)b;
+    b += n * sizeof(uint64_t);
+    h->P.lanes = (ldpcp_lane*)b;
+    b += bytes_lanes;
+    h->P.win = (uint64_t*)b;
+    b += bytes_win;
+    h->P.wbase = (int32_t*)b;
+    b += bytes_wbase;
+    b = (char*)(((uintptr_t)b + 255) & ~(uintptr_t)255);
+    h->P.frames = (double*)b;
+    h->P.g = h->P.f = h->P.c = nullptr;
+    h->P.binit = nullptr;
+    e = hipStreamCreate(&h->stream);
+    if (e != hipSuccess) {
+        set_error("ldpcp_run_create: hipStreamCreate failed: %s", hipGetErrorString(e));
+        (void)hipFree(h->state_block);
+        delete h;
+        return LDPCP_EDEVICE;
+    }
+    *out = h;
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_set_replay(ldpcp_run* h, const double* gauss, uint64_t n_gauss, const double* flip,
+                                    uint64_t n_flip, const double* catchup, uint64_t catch_cap, const int8_t* binit,
+                                    int32_t bmin, int32_t bn)
+{
+    if (!h || !gauss || (!flip && n_flip) || (!catchup && catch_cap) || (!binit && bn)) {
+        set_error("ldpcp_run_set_replay: null argument");
+        return LDPCP_EINVAL;
+    }
+    const uint64_t n = h->c.n_traj;
+    const uint64_t need_g = h->c.init_gauss + h->c.nsteps_eff;
+    const uint64_t need_f = h->c.init_flip + (h->c.tau_zero ? 0 : h->c.nsteps_eff);
+    if (n_gauss < need_g || n_flip < need_f) {
+        set_error("ldpcp_run_set_replay: streams too short (gauss %llu < %llu or flip %llu < %llu)",
+                  (unsigned long long)n_gauss, (unsigned long long)need_g, (unsigned long long)n_flip,
+                  (unsigned long long)need_f);
+        return LDPCP_ERANGE;
+    }
+    const uint64_t bytes = (n_gauss + n_flip) * n * sizeof(double) + n * catch_cap * sizeof(double) +
+                           2 * n * (uint64_t)bn + 256;
+    void* block = nullptr;
+    LDPCP_HIP_TRY(hipMalloc(&block, bytes));
+    char* b = (char*)block;
+    double* g = (double*)b;
+    b += n_gauss * n * sizeof(double);
+    double* f = (double*)b;
+    b += n_flip * n * sizeof(double);
+    double* c = (double*)b;
+    b += n * catch_cap * sizeof(double);
+    int8_t* bi = (int8_t*)b;
+    LDPCP_HIP_TRY(hipMemcpy(g, gauss, n_gauss * n * sizeof(double), hipMemcpyHostToDevice));
+    if (n_flip) LDPCP_HIP_TRY(hipMemcpy(f, flip, n_flip * n * sizeof(double), hipMemcpyHostToDevice));
+    if (catch_cap) LDPCP_HIP_TRY(hipMemcpy(c, catchup, n * catch_cap * sizeof(double), hipMemcpyHostToDevice));
+    if (bn) LDPCP_HIP_TRY(hipMemcpy(bi, binit, 2 * n * (uint64_t)bn, hipMemcpyHostToDevice));
+    h->P.g = g;
+    h->P.f = f;
+    h->P.c = c;
+    h->P.binit = bi;
+    h->c.replay = 1;
+    h->c.n_gauss = n_gauss;
+    h->c.n_flip = n_flip;
+    h->c.catch_cap = catch_cap;
+    h->c.bmin = bmin;
+    h->c.bn = bn;
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_integrate(ldpcp_run* h, uint64_t steps_per_launch)
+{
+    if (!h) {
+        set_error("ldpcp_run_integrate: null handle");
+        return LDPCP_EINVAL;
+    }
+    if (steps_per_launch == 0) steps_per_launch = 1ull << 20;
+    const uint64_t n = h->c.n_traj;
+    LDPCP_HIP_TRY(hipMemsetAsync(h->P.lanes, 0, n * sizeof(ldpcp_lane), h->stream));
+    LDPCP_HIP_TRY(hipMemsetAsync(h->P.win, 0, n * 2 * (uint64_t)h->c.wb * sizeof(uint64_t), h->stream));
+    const uint64_t fields = h->params.save_all ? kFields : 1;
+    LDPCP_HIP_TRY(hipMemsetAsync(h->P.frames, 0, fields * h->c.frames_cap * n * sizeof(double), h->stream));
+    const dim3 block(256), grid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(run_init_kernel, grid, block, 0, h->stream, h->c, h->P);
+    LDPCP_HIP_TRY(hipGetLastError());
+    for (uint64_t s0 = 0; s0 < h->c.nsteps_eff; s0 += steps_per_launch) {
+        const uint64_t s1 = (s0 + steps_per_launch < h->c.nsteps_eff) ? s0 + steps_per_launch : h->c.nsteps_eff;
+        hipLaunchKernelGGL(run_step_kernel, grid, block, 0, h->stream, h->c, h->P, s0, s1);
+        LDPCP_HIP_TRY(hipGetLastError());
+    }
+    LDPCP_HIP_TRY(hipStreamSynchronize(h->stream));
+    h->integrated = true;
+    // surface the first per-lane error
+    std::vector<ldpcp_lane> lanes(n);
+    LDPCP_HIP_TRY(hipMemcpy(lanes.data(), h->P.lanes, n * sizeof(ldpcp_lane), hipMemcpyDeviceToHost));
+    for (uint64_t t = 0; t < n; ++t) {
+        if (lanes[t].error) {
+            const char* what = lanes[t].error == 2   ? "replay barrier table does not cover a touched barrier"
+                               : lanes[t].error == 3 ? "trajectory left its barrier window (raise barrier_window)"
+                                                     : "replay catch-up stream exhausted";
+            set_error("ldpcp_run_integrate: trajectory %llu: %s", (unsigned long long)t, what);
+            return LDPCP_ERANGE;
+        }
+    }
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_lanes(ldpcp_run* h, uint32_t begin, uint32_t n, ldpcp_lane* out)
+{
+    if (!h || !out || (uint64_t)begin + n > h->c.n_traj) {
+        set_error("ldpcp_run_lanes: bad handle, output or range");
+        return LDPCP_EINVAL;
+    }
+    LDPCP_HIP_TRY(hipMemcpy(out, h->P.lanes + begin, (uint64_t)n * sizeof(ldpcp_lane), hipMemcpyDeviceToHost));
+    std::vector<double> buf((uint64_t)n);
+    struct Col {
+        const double* src;
+        size_t off;
+    } cols[] = {{h->P.x, offsetof(ldpcp_lane, x)},           {h->P.v, offsetof(ldpcp_lane, v)},
+                {h->P.F, offsetof(ldpcp_lane, f)},           {h->P.E, offsetof(ldpcp_lane, e)},
+                {h->P.R, offsetof(ldpcp_lane, r_prev)},      {h->P.state, offsetof(ldpcp_lane, state)},
+                {h->P.barrier, offsetof(ldpcp_lane, barrier)}, {h->P.well, offsetof(ldpcp_lane, well)},
+                {h->P.periodic, offsetof(ldpcp_lane, periodic)}, {h->P.xp, offsetof(ldpcp_lane, xp)},
+                {h->P.local, offsetof(ldpcp_lane, local_barrier)}};
+    for (const Col& col : cols) {
+        LDPCP_HIP_TRY(hipMemcpy(buf.data(), col.src + begin, (uint64_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) memcpy((char*)(out + i) + col.off, &buf[i], sizeof(double));
+    }
+    std::vector<uint64_t> cnt((uint64_t)n);
+    for (int s = 0; s < 2; ++s) {
+        LDPCP_HIP_TRY(hipMemcpy(cnt.data(), (s ? h->P.cnt1 : h->P.cnt0) + begin, (uint64_t)n * sizeof(uint64_t),
+                                hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) out[i].state_count[s] = (double)cnt[i];
+    }
+    return LDPCP_OK;
+}
+
+extern "C" uint64_t ldpcp_run_n_frames(const ldpcp_run* h) { return h ? h->n_frames : 0; }
+
+extern "C" int ldpcp_run_frames(ldpcp_run* h, int32_t field, uint32_t begin, uint32_t n, double* out)
+{
+    if (!h || !out || (uint64_t)begin + n > h->c.n_traj || field < 0 || field >= kFields ||
+        (field > 0 && !h->params.save_all)) {
+        set_error("ldpcp_run_frames: bad handle, field or range");
+        return LDPCP_EINVAL;
+    }
+    const uint64_t nf = h->n_frames, nt = h->c.n_traj;
+    std::vector<double> plane(nf * nt);
+    LDPCP_HIP_TRY(hipMemcpy(plane.data(), h->P.frames + (uint64_t)field * nf * nt, nf * nt * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint64_t f = 0; f < nf; ++f) out[(uint64_t)i * nf + f] = plane[f * nt + begin + i];
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_msd(ldpcp_run* h, const uint64_t* lags, uint32_t nlags, double* out)
+{
+    if (!h || !lags || !out || nlags == 0 || nlags > 65535) {
+        set_error("ldpcp_run_msd: bad arguments");
+        return LDPCP_EINVAL;
+    }
+    const uint64_t nt = h->c.n_traj;
+    uint64_t* d_lags = nullptr;
+    double* d_out = nullptr;
+    LDPCP_HIP_TRY(hipMalloc(&d_lags, nlags * sizeof(uint64_t)));
+    LDPCP_HIP_TRY(hipMalloc(&d_out, nt * nlags * sizeof(double)));
+    LDPCP_HIP_TRY(hipMemcpy(d_lags, lags, nlags * sizeof(uint64_t), hipMemcpyHostToDevice));
+    const dim3 block(256), grid((unsigned)((nt + 255) / 256), nlags);
+    hipLaunchKernelGGL(msd_kernel, grid, block, 0, h->stream, h->P.frames, h->n_frames, (uint32_t)nt, d_lags, nlags,
+                       d_out);
+    LDPCP_HIP_TRY(hipGetLastError());
+    LDPCP_HIP_TRY(hipStreamSynchronize(h->stream));
+    LDPCP_HIP_TRY(hipMemcpy(out, d_out, nt * nlags * sizeof(double), hipMemcpyDeviceToHost));
+    (void)hipFree(d_lags);
+    (void)hipFree(d_out);
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_destroy(ldpcp_run* h)
+{
+    if (!h) return LDPCP_OK;
+    if (h->c.replay) (void)hipFree((void*)h->P.g);
+    (void)hipFree(h->state_block);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return LDPCP_OK;
+}

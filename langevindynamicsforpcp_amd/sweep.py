@@ -1,0 +1,94 @@
+"""Host side of the batched first-passage engine (ldpcp_sweep_* in include/ldpcp.h)."""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _native as N
+
+
+def sweep_set(model=0, overdamped=True, eb=2.0, ratio=3.6, n_traj=1 << 20, max_steps=10 ** 8, seed=1000,
+              x0=-2.9, periodic_length=5.8, half_width=(0.65, 0.65), position=(0.0, 0.0), timestep=1e-4,
+              tau=1e-5, gamma=1.0, kb=1.0, temperature=1.0, mass=1.0, heights=None,
+              randomize_state=True, randomize_v=False, v0=0.0, equal_probability=True, eq_constant=1.0):
+    """A parameter set.  By default the barrier alternates between ratio*Eb and
+    Eb (BARRIERHEIGHTS ratio*Eb Eb, as in sample_input/CuIDB_single)."""
+    s = N.SweepSet()
+    s.model = model
+    s.overdamped = int(overdamped)
+    s.x0, s.v0, s.gamma, s.kb, s.temperature, s.mass = x0, v0, gamma, kb, temperature, mass
+    s.randomize_v = int(randomize_v)
+    s.randomize_state = int(randomize_state)
+    s.initial_state = 0.0
+    s.periodic_length = periodic_length
+    if heights is None:
+        heights = [[ratio * eb, eb], [ratio * eb, eb]]
+    for b in range(2):
+        for k in range(2):
+            s.height[b][k] = heights[b][k]
+        s.half_width[b] = half_width[b]
+        s.position[b] = position[b]
+    s.timestep, s.tau = timestep, tau
+    s.equal_probability = int(equal_probability)
+    s.eq_constant = eq_constant
+    s.n_traj, s.max_steps, s.seed = int(n_traj), int(max_steps), int(seed)
+    return s
+
+
+class Sweep:
+    """Absorbing first-passage runs over a list of parameter sets on the GPU."""
+
+    def __init__(self, sets, precision=32):
+        self.sets = list(sets)
+        arr = (N.SweepSet * len(self.sets))(*self.sets)
+        lib = N.load()
+        h = C.c_void_p()
+        N.check(lib.ldpcp_sweep_create(arr, len(self.sets), int(precision), C.byref(h)), "ldpcp_sweep_create")
+        self._h = h
+        self.precision = precision
+
+    def launch(self, steps_per_launch):
+        live = C.c_uint64()
+        N.check(N.load().ldpcp_sweep_launch(self._h, int(steps_per_launch), C.byref(live)), "ldpcp_sweep_launch")
+        return live.value
+
+    def run(self, steps_per_launch=4096):
+        N.check(N.load().ldpcp_sweep_run(self._h, int(steps_per_launch)), "ldpcp_sweep_run")
+
+    def stats(self, i):
+        st = N.SetStats()
+        N.check(N.load().ldpcp_sweep_stats(self._h, int(i), C.byref(st)), "ldpcp_sweep_stats")
+        return st
+
+    def fpt_steps(self, i):
+        n = self.sets[i].n_traj
+        a = np.zeros(n, np.uint64)
+        t = np.zeros(n, np.uint64)
+        N.check(N.load().ldpcp_sweep_fpt(self._h, int(i), a.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         t.ctypes.data_as(C.POINTER(C.c_uint64))), "ldpcp_sweep_fpt")
+        return a, t
+
+    def kernel_time(self):
+        ms = C.c_double()
+        n = C.c_uint64()
+        N.check(N.load().ldpcp_sweep_kernel_time(self._h, C.byref(ms), C.byref(n)), "ldpcp_sweep_kernel_time")
+        return ms.value, n.value
+
+    def mfpt(self, i):
+        st = self.stats(i)
+        if st.n_absorbed == 0:
+            return math.nan, math.nan
+        m = st.fpt_sum / st.n_absorbed
+        var = max(st.fpt_sum2 / st.n_absorbed - m * m, 0.0)
+        return m, math.sqrt(var / st.n_absorbed)
+
+    def close(self):
+        if self._h is not None:
+            N.load().ldpcp_sweep_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
