@@ -40,6 +40,12 @@ def workload_set(n_traj, seed=1000):
 def time_engine(precision, n_traj, steps, warmup, spl):
     from langevindynamicsforpcp_amd.sweep import Sweep
     sw = Sweep([workload_set(n_traj)], precision=precision)
+    # the first launches end on the dense early MSD lag steps (1, 2, 3, ...);
+    # run through that prefix before the warmup proper
+    steps_done = 0
+    while steps_done < 2 * spl:
+        sw.launch(spl)
+        steps_done = sw.stats(0).traj_steps // n_traj
     for _ in range(warmup):
         sw.launch(spl)
     st0 = sw.stats(0).traj_steps
