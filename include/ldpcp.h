@@ -51,6 +51,7 @@ typedef struct ldpcp_run_params {
     int32_t save_all;              /* WRITEX: keep v, f, pe, ke, state, barrier frames */
     uint64_t seed;                 /* Philox key (free-running noise) */
     int32_t barrier_window;        /* barrier entries kept per trajectory and barrier array */
+    uint64_t tick_capacity;        /* barrier-crossing ticks kept per trajectory (0 = none) */
 } ldpcp_run_params;
 
 /* Per-trajectory results: the values the runner reads through
@@ -109,6 +110,12 @@ int ldpcp_run_lanes(ldpcp_run* h, uint32_t begin, uint32_t n, ldpcp_lane* out);
  * 4 ke, 5 state, 6 barrier.  out is [n][n_frames]. */
 uint64_t ldpcp_run_n_frames(const ldpcp_run* h);
 int ldpcp_run_frames(ldpcp_run* h, int32_t field, uint32_t begin, uint32_t n, double* out);
+
+/* Replaces GetBarrierTick (langevinDynamicsSingleBarrier.h:2702-2704): the
+ * waiting-time barrier ticks (in steps) of trajectories [begin, begin+n),
+ * out is [n][tick_capacity]; the count per trajectory is ldpcp_lane.wt_ticks
+ * (a count above tick_capacity means the list was truncated). */
+int ldpcp_run_ticks(ldpcp_run* h, uint32_t begin, uint32_t n, double* out);
 
 /* Replaces MeanSquareDisplacementBruteForce over each lane's saved x frames
  * (langevinDynamicsSingleBarrier.h:24-45, called at :3454).  out is

@@ -38,6 +38,7 @@ class RunParams(C.Structure):
         ("save_all", C.c_int32),
         ("seed", C.c_uint64),
         ("barrier_window", C.c_int32),
+        ("tick_capacity", C.c_uint64),
     ]
 
 
@@ -107,6 +108,7 @@ _SIGNATURES = [
     ("ldpcp_run_lanes", C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(LaneResult)]),
     ("ldpcp_run_n_frames", C.c_uint64, [_P]),
     ("ldpcp_run_frames", C.c_int, [_P, C.c_int32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
+    ("ldpcp_run_ticks", C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
     ("ldpcp_run_msd", C.c_int, [_P, C.POINTER(C.c_uint64), C.c_uint32, C.POINTER(C.c_double)]),
     ("ldpcp_run_destroy", C.c_int, [_P]),
     ("ldpcp_sweep_create", C.c_int, [C.POINTER(SweepSet), C.c_int32, C.c_int32, C.POINTER(_P)]),

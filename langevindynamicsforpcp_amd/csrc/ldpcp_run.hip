@@ -39,7 +39,7 @@ struct RunConst {
     double vsigma, x0, v0, initial_state;
     uint64_t save_freq, nsteps_eff, frames_cap;
     uint32_t key0, key1, n_traj, traj_offset;
-    uint64_t n_gauss, n_flip, catch_cap, init_gauss, init_flip;
+    uint64_t n_gauss, n_flip, catch_cap, init_gauss, init_flip, tick_cap;
     int32_t bmin, bn;
 };
 
@@ -54,6 +54,7 @@ struct RunPtrs {
     const double* f;     // replay flip uniforms [k][traj]
     const double* c;     // replay catch-up uniforms [traj][cap]
     const int8_t* binit; // replay barrier states [array][traj][bn]
+    double* ticks;       // barrier ticks [traj][tick_cap]
 };
 
 // Constants of IntegrateInitializeVariables (langevinDynamicsSingleBarrier.h:1520-1618,
@@ -151,6 +152,7 @@ void make_const(const ldpcp_run_params& p, RunConst& c)
     c.key1 = (uint32_t)(p.seed >> 32);
     c.init_gauss = (uint64_t)(p.randomize_v ? 1 : 0) + (uint64_t)(p.overdamped ? 1 : 0);
     c.init_flip = p.randomize_state ? 1 : 0;
+    c.tick_cap = p.tick_capacity;
 }
 
 // ---------------------------------------------------------------------------
@@ -411,7 +413,7 @@ __device__ __forceinline__ double ev(const RunConst& c, uint64_t iStep, double l
     return fma(c.dt, (double)iStep, -last);
 }
 
-__device__ void stats_event(const RunConst& c, ldpcp_lane* o, const Lane& s, uint64_t iStep)
+__device__ void stats_event(const RunConst& c, ldpcp_lane* o, const Lane& s, uint64_t iStep, double* ticks)
 {
     const double t = c.dt * (double)iStep;
     const bool single = (c.model == 0);
@@ -421,6 +423,7 @@ __device__ void stats_event(const RunConst& c, ldpcp_lane* o, const Lane& s, uin
             o->wt_passed = 1;
         } else if (s.ent_period && o->wt_passed) {
             const double tick = o->wt_tick_prev;
+            if (o->wt_ticks < c.tick_cap) ticks[o->wt_ticks] = tick;
             if (o->wt_ticks > 0) {
                 const double w = (tick - o->wt_last) * c.dt;
                 o->wt_sum += w;
@@ -691,7 +694,7 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         lane_param(s, c);
         lane_energy_force(s, c);
         if (!c.overdamped) s.v = s.v + (c.minv * s.F) * c.halfdt;
-        if (s.ent_period || s.ent_well) stats_event(c, o, s, iStep);
+        if (s.ent_period || s.ent_well) stats_event(c, o, s, iStep, P.ticks + (uint64_t)t * c.tick_cap);
         if (iStep % c.save_freq == 0) record(c, P, t, s, iStep / c.save_freq);
     }
     o->n_catch = n_catch;
@@ -795,7 +798,8 @@ extern "C" int ldpcp_run_create(const ldpcp_run_params* p, uint32_t n_traj, uint
     const uint64_t bytes_win = n * 2 * (uint64_t)p->barrier_window * sizeof(uint64_t);
     const uint64_t bytes_wbase = n * sizeof(int32_t);
     const uint64_t bytes_frames = fields * h->c.frames_cap * n * sizeof(double);
-    const uint64_t total = bytes_soa + bytes_lanes + bytes_win + bytes_wbase + bytes_frames + 1024;
+    const uint64_t bytes_ticks = n * p->tick_capacity * sizeof(double);
+    const uint64_t total = bytes_soa + bytes_lanes + bytes_win + bytes_wbase + bytes_frames + bytes_ticks + 1024;
     hipError_t e = hipMalloc(&h->state_block, total);
     if (e != hipSuccess) {
         set_error("ldpcp_run_create: hipMalloc(%llu bytes) failed: %s", (unsigned long long)total, hipGetErrorString(e));
@@ -821,6 +825,8 @@ extern "C" int ldpcp_run_create(const ldpcp_run_params* p, uint32_t n_traj, uint
     b += bytes_wbase;
     b = (char*)(((uintptr_t)b + 255) & ~(uintptr_t)255);
     h->P.frames = (double*)b;
+    b += bytes_frames;
+    h->P.ticks = (double*)b;
     h->P.g = h->P.f = h->P.c = nullptr;
     h->P.binit = nullptr;
     e = hipStreamCreate(&h->stream);
@@ -962,6 +968,17 @@ extern "C" int ldpcp_run_frames(ldpcp_run* h, int32_t field, uint32_t begin, uin
                             hipMemcpyDeviceToHost));
     for (uint32_t i = 0; i < n; ++i)
         for (uint64_t f = 0; f < nf; ++f) out[(uint64_t)i * nf + f] = plane[f * nt + begin + i];
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_ticks(ldpcp_run* h, uint32_t begin, uint32_t n, double* out)
+{
+    if (!h || !out || (uint64_t)begin + n > h->c.n_traj || h->c.tick_cap == 0) {
+        set_error("ldpcp_run_ticks: bad handle, range, or no tick capacity");
+        return LDPCP_EINVAL;
+    }
+    LDPCP_HIP_TRY(hipMemcpy(out, h->P.ticks + (uint64_t)begin * h->c.tick_cap,
+                            (uint64_t)n * h->c.tick_cap * sizeof(double), hipMemcpyDeviceToHost));
     return LDPCP_OK;
 }
 

@@ -1,0 +1,610 @@
+// runner.cpp -- see runner.h.  Output formats follow
+// langevinDynamicsSingleBarrier.h:3040-3124 (headers) and :3594-4036 (files).
+#include "runner.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <time.h>
+
+#include <algorithm>
+#include <array>
+#include <string.h>
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "ldpcp.h"
+#include "text_util.h"
+
+namespace ldpcp_host {
+
+namespace {
+
+// HISTOGRAM::HISTOGRAM (clsHistogram.cpp:96-263): AddValue truncates
+// (v - min) / step to int; HistogramVector never counts (its loop bounds are
+// taken before `last = -1` is clamped, clsHistogram.cpp:242-254), so the
+// barrier-height and waiting-time histograms the runner fills through it stay 0.
+struct Histogram {
+    double min = 0, max = 0, step = 1, step_inv = 1;
+    int size = 0;
+    std::vector<double> count, percent;
+    void allocate(double lo, double hi, int n)
+    {
+        min = lo;
+        max = hi;
+        size = n;
+        step = (hi - lo) / n;
+        step_inv = 1.0 / step;
+        count.assign(n, 0.0);
+        percent.assign(n, 0.0);
+    }
+    void add(double v)
+    {
+        const int index = (int)((v - min) * step_inv);
+        if (index >= 0 && index < size) count[index]++;
+    }
+    void normalize()
+    {
+        double sum = 0.0;
+        for (int i = 0; i < size; ++i) sum += count[i];
+        if (sum != 0.0)
+            for (int i = 0; i < size; ++i) percent[i] = count[i] / sum;
+    }
+    void print(FILE* f) const
+    {
+        fprintf(f, "# HMin %23.16E\n", min);
+        fprintf(f, "# HMax %23.16E\n", max);
+        fprintf(f, "# HBin %23.16E\n", step);
+        fprintf(f, "# NBin %23d\n", size);
+        for (int i = 0; i < size; ++i)
+            fprintf(f, "%23.16E %23.16E %23.16E %23.0f %23.16E\n", fma(step, (double)i, min),
+                    fma(step, i + 0.5, min), fma(step, i + 1.0, min), count[i], percent[i]);
+    }
+};
+
+// STATISTICS::AVERAGE / RMSD (STATISTICS.cpp:100-134); the squared terms are
+// one statement each, contracted by the reference's compiler.
+double average(const std::vector<double>& v)
+{
+    if (v.empty()) return 0.0;
+    double s = 0.0;
+    for (double x : v) s += x;
+    return s / (double)v.size();
+}
+
+double rmsd(const std::vector<double>& v)
+{
+    if (v.empty()) return 0.0;
+    const double a = average(v);
+    double r = 0.0;
+    for (double x : v) r = fma(x - a, x - a, r);
+    return sqrt(r / (double)v.size());
+}
+
+double xc_of(const RunnerConfig& c, double* yc)
+{
+    // LangevinEquationSIMDMultiple::GetXc (langevinDynamicsDoubleBarrier.h:1722-1732)
+    const double x1b = c.position[0], w1 = c.half_width[0], x2b = c.position[1], w2 = c.half_width[1];
+    const double y1 = c.height[0][0], y2 = c.height[1][0];
+    const double x1h = x1b + w1, x2l = x2b - w2;
+    const double xc = fma(x1h * y1, x2b - x2l, x2l * y2 * (x1h - x1b)) / fma(y1, x2b - x2l, y2 * (x1h - x1b));
+    *yc = (x1h - xc) / (x1h - x1b) * y1;
+    return xc;
+}
+
+struct Header {
+    const RunnerConfig* c;
+    int nrun = 0;
+    int seed_used = -1;
+    double traj_sets = 0, memory = 0, save_dt = 0;
+
+    // PrintParameter (Single.h:3040-3124; Double.h lines for [b][s] heights, Xc/Yc, positions)
+    void print(FILE* f) const
+    {
+        const RunnerConfig& r = *c;
+        const char* bar = "# ==============================================\n";
+        fprintf(f, "%s", bar);
+        fprintf(f, "# Overdamped           = %13s\n", bool_text(r.overdamped));
+        fprintf(f, "%s", bar);
+        fprintf(f, "# Number of node       = %13d\n", r.ranks);
+        fprintf(f, "%s", bar);
+        fprintf(f, "# Time step            = %13.7f\n", r.timestep);
+        fprintf(f, "# Save frequency       = %13zd\n", (size_t)r.save_freq);
+        fprintf(f, "# Save time step       = %13.7f\n", save_dt);
+        fprintf(f, "# Statistics freq      = %13zd\n", (size_t)r.stat_freq);
+        fprintf(f, "# Number of step (pre) = %13zd\n", (size_t)r.nsteps_previous);
+        fprintf(f, "# Number of step       = %13zd\n", (size_t)r.nsteps);
+        fprintf(f, "# Simulation time      = %13.6E\n", r.timestep * (double)r.nsteps);
+        fprintf(f, "# Number of trial      = %13d\n", r.ntrial);
+        fprintf(f, "# Number of run/trial  = %13d\n", r.simd_width);
+        fprintf(f, "# Number of run        = %13d\n", nrun);
+        fprintf(f, "%s", bar);
+        if (r.model == 0) {
+            fprintf(f, "# Barrier height [0]   = %13.7f\n", r.height[0][0]);
+            fprintf(f, "# Barrier height [1]   = %13.7f\n", r.height[0][1]);
+            fprintf(f, "# Barrier periodicity  = %13.7f\n", r.periodic_length);
+            fprintf(f, "# slope half length    = %13.7f\n", r.half_width[0]);
+        } else {
+            double yc = 0.0;
+            const double xc = xc_of(r, &yc);
+            fprintf(f, "# Barrier height [0][0]= %13.7f\n", r.height[0][0]);
+            fprintf(f, "# Barrier height [0][1]= %13.7f\n", r.height[0][1]);
+            fprintf(f, "# Barrier height [1][0]= %13.7f\n", r.height[1][0]);
+            fprintf(f, "# Barrier height [1][1]= %13.7f\n", r.height[1][1]);
+            fprintf(f, "# Xc                   = %13.7f\n", xc);
+            fprintf(f, "# Yc                   = %13.7f\n", yc);
+            fprintf(f, "# Barrier periodicity  = %13.7f\n", r.periodic_length);
+            fprintf(f, "# Slope half length[0] = %13.7f\n", r.half_width[0]);
+            fprintf(f, "# Slope half length[1] = %13.7f\n", r.half_width[1]);
+            fprintf(f, "# Barrier position [0] = %13.7f\n", r.position[0]);
+            fprintf(f, "# Barrier position [1] = %13.7f\n", r.position[1]);
+        }
+        fprintf(f, "# Equal probability    = %13s\n", bool_text(r.equal_probability));
+        fprintf(f, "# Equilibrium constant = %13.7E\n", r.eq_constant);
+        fprintf(f, "%s", bar);
+        fprintf(f, "# Mass                 = %13.7f\n", r.mass);
+        fprintf(f, "# Gamma                = %13.7f\n", r.gamma);
+        fprintf(f, "# Temperature          = %13.7f\n", r.temperature);
+        fprintf(f, "# kB                   = %13.7f\n", r.kb);
+        fprintf(f, "# Eta variance (Q)     = %13.7f\n", r.q);
+        fprintf(f, "# Eta tau              = %13.7f\n", r.tau);
+        fprintf(f, "# Sample per order     = %13d\n", r.sample_per_order);
+        fprintf(f, "%s", bar);
+        fprintf(f, "# X0                   = %13.7f\n", r.x0);
+        if (r.randomize_v) fprintf(f, "# V0                   = %13s\n", "Random");
+        else fprintf(f, "# V0                   = %13.7f\n", r.v0);
+        fprintf(f, "# Eta0                 = %13s\n", "Random");
+        if (r.randomize_state) fprintf(f, "# State0               = %13s\n", "Random");
+        else fprintf(f, "# State0               = %13d\n", (int)r.initial_state);
+        fprintf(f, "# Same initial barrier = %13s\n", bool_text(r.same_initial_barrier));
+        fprintf(f, "%s", bar);
+        fprintf(f, "# Random number seed0  = %13d\n", r.seed);
+        fprintf(f, "# Random number seed   = %13d\n", seed_used);
+        fprintf(f, "%s", bar);
+        fprintf(f, "# Number of traj(each) = %13.0f\n", traj_sets);
+        fprintf(f, "# Memory usage (each)  = %13s\n", bytes_readable(memory).c_str());
+        fprintf(f, "# Memory usage (total) = %13s\n", bytes_readable(memory * r.ranks).c_str());
+        fprintf(f, "%s", bar);
+        fprintf(f, "# MSD                  = %13s\n", "Mandatory");
+        fprintf(f, "# Waiting time         = %13s\n", bool_text(r.waiting));
+        fprintf(f, "# Write trajectories   = %13s\n", bool_text(r.write_x));
+        fprintf(f, "# V0 Distribution      = %13s\n", bool_text(r.v0_distribution));
+        fprintf(f, "# Eta0 Distribution    = %13s\n", bool_text(false));
+        fprintf(f, "# State0 Distribution  = %13s\n", bool_text(r.state0_distribution));
+        fprintf(f, "# Eta result           = %13s\n", bool_text(false));
+        fprintf(f, "# First passage time   = %13s\n", bool_text(r.fpt));
+        if (r.fpt) {
+            fprintf(f, "# Barrier hist hmin    = %13.7f\n", 0.0);
+            fprintf(f, "# Barrier hist hmax    = %13.7f\n", 10.0);
+            fprintf(f, "# Barrier hist nbin    = %13d\n", 100);
+        }
+        fprintf(f, "# Transtion path time  = %13s\n", bool_text(r.tpt));
+        fprintf(f, "# First passage time CG= %13s\n", bool_text(r.fptcg));
+        if (r.fptcg)
+            for (size_t j = 0; j < r.cg_size.size(); ++j)
+                fprintf(f, "# FFT CG size %4d     = %13d\n", (int)j, r.cg_size[j]);
+        fprintf(f, "# X correlation        = %13s\n", bool_text(false));
+        fprintf(f, "%s", bar);
+        fprintf(f, "# Read restart         = %13s\n", bool_text(r.read_restart));
+        fprintf(f, "# Write restart        = %13s\n", bool_text(r.write_restart));
+        fprintf(f, "# Input restart (base) = %13s\n", r.input_restart.c_str());
+        fprintf(f, "# Output restart (base)= %13s\n", r.output_restart.c_str());
+        fprintf(f, "%s", bar);
+    }
+};
+
+FILE* open_output(const std::string& name)
+{
+    backup_existing(name);
+    FILE* f = fopen(name.c_str(), "w+");
+    if (!f) fprintf(stderr, "Error: can't open output file (%s)\n", name.c_str());
+    return f;
+}
+
+void write_dat(const std::string& name, const std::vector<double>& v)
+{
+    backup_existing(name);
+    FILE* f = fopen(name.c_str(), "wb");
+    if (!f) {
+        fprintf(stderr, "Error: can't open output file (%s)\n", name.c_str());
+        return;
+    }
+    if (!v.empty()) fwrite(v.data(), sizeof(double), v.size(), f);
+    fclose(f);
+}
+
+void write_mean_log(const std::string& name, const Header& h, const std::vector<double>& v)
+{
+    FILE* f = open_output(name);
+    if (!f) return;
+    h.print(f);
+    fprintf(f, "# average %23.16E\n", average(v));
+    fprintf(f, "# sd      %23.16E\n", rmsd(v));
+    fprintf(f, "# count   %23zd\n", v.size());
+    fclose(f);
+}
+
+[[noreturn]] void die(const char* what)
+{
+    fprintf(stderr, "Error: %s: %s\n", what, ldpcp_last_error());
+    exit(1);
+}
+
+}  // namespace
+
+std::vector<int> rank_seeds(int master_seed, int ranks, int* used_master)
+{
+    const int value = master_seed == -1 ? (int)time(nullptr) : master_seed;
+    srand(value);
+    if (used_master) *used_master = value;
+    std::vector<int> out;
+    while ((int)out.size() < ranks) {
+        const int v = rand() % RAND_MAX;
+        if (std::find(out.begin(), out.end(), v) == out.end()) out.push_back(v);
+    }
+    return out;
+}
+
+std::vector<unsigned long> log10_scale(double timestep, unsigned long n_frames, unsigned long minimal,
+                                       unsigned long per_order, bool include_zero)
+{
+    const double step = 1.0 / per_order;
+    const double max_time = timestep * (double)(n_frames - minimal);
+    const double t0 = log10(timestep), t1 = log10(max_time);
+    std::vector<unsigned long> out;
+    if (include_zero) out.push_back(0);
+    for (double t = t0; t <= t1; t += step) {
+        const unsigned long iv = (unsigned long)round(pow(10.0, t) / timestep);
+        if (out.empty() || out.back() != iv) out.push_back(iv);
+    }
+    return out;
+}
+
+int run(const RunnerConfig& cfg)
+{
+    using clock = std::chrono::steady_clock;
+    const auto t_start = clock::now();
+    fprintf(stderr, "LangevinEquationMultipleRunner\n");
+    const int W = cfg.simd_width, R = cfg.ranks;
+    const unsigned long n_frames = (cfg.nsteps + cfg.nsteps_previous) / cfg.save_freq + 1;
+    if (n_frames < 1000) {
+        fprintf(stderr, "Error: numberOfFrame < 1000\n");
+        return 0;
+    }
+    if (cfg.read_restart || cfg.nsteps_previous != 0) {
+        fprintf(stderr, "Error: can't read restart file on rank %10d (restart input is not supported by this build)\n",
+                0);
+        return 0;
+    }
+    Header hd;
+    hd.c = &cfg;
+    hd.nrun = cfg.ntrial * W * R;
+    const std::vector<int> seeds = rank_seeds(cfg.seed, R, nullptr);
+    hd.seed_used = seeds[0];
+    hd.traj_sets = cfg.write_x ? 7 : 1;
+    hd.memory = hd.traj_sets * (double)(cfg.nsteps / cfg.save_freq + 1) * sizeof(double) * W;
+    hd.save_dt = cfg.timestep * (double)cfg.save_freq;
+    hd.print(stderr);
+
+    const std::vector<unsigned long> interval = log10_scale(hd.save_dt, n_frames, 1000, cfg.sample_per_order, true);
+    if (interval.size() < 20) {
+        fprintf(stderr, "Error: interval is less than 20 (%zd)\n", interval.size());
+        exit(0);
+    } else if (interval.size() < 100) {
+        fprintf(stderr, "Warning: interval is less than 100 (%zd)\n", interval.size());
+    } else {
+        fprintf(stderr, "Interval = %10zd\n", interval.size());
+    }
+    if (cfg.write_restart)
+        fprintf(stderr, "Warning: WRITERESTART is not supported by this build; no restart file is written\n");
+
+    const size_t nl = interval.size();
+    const size_t ncg = cfg.fptcg ? cfg.cg_size.size() : 0;
+    // per emulated rank, as each MPI rank accumulates them (Single.h:3339-3414)
+    std::vector<std::vector<double>> msd(R, std::vector<double>(nl, 0.0));
+    std::vector<std::array<double, 2>> state_count(R, {0.0, 0.0});
+    std::vector<std::vector<double>> fpt(R), fpt2(R), tpt(R), tpt2(R);
+    std::vector<std::vector<std::vector<double>>> cg(ncg, std::vector<std::vector<double>>(R));
+    std::vector<double> wt_sum(R, 0.0), wt_sum2(R, 0.0), wt_max(R, 0.0), wt_n(R, 0.0);
+    std::vector<Histogram> h_v0(R), h_s0(R);
+    const double vsigma = sqrt(cfg.kb * cfg.temperature / cfg.mass);
+    for (int r = 0; r < R; ++r) {
+        if (cfg.v0_distribution) h_v0[r].allocate(-vsigma * 5.0, vsigma * 5.0, 200);
+        if (cfg.state0_distribution) h_s0[r].allocate(0.0, 2.0, 2);
+    }
+    Histogram h_init, h_inst;
+    if (cfg.fpt) {
+        h_init.allocate(0.0, 10.0, 100);
+        h_inst.allocate(0.0, 10.0, 100);
+    }
+    const uint64_t tick_cap = cfg.waiting ? std::min<uint64_t>(1u << 16, (1ull << 27) / (uint64_t)W + 1) : 0;
+    bool wt_exact = true;
+    std::vector<std::vector<double>> wt_list(R);
+    double t_integrate = 0.0, t_analysis = 0.0;
+
+    ldpcp_run_params p;
+    memset(&p, 0, sizeof(p));
+    p.model = cfg.model;
+    p.overdamped = cfg.overdamped;
+    p.x0 = cfg.x0;
+    p.v0 = cfg.v0;
+    p.gamma = cfg.gamma;
+    p.kb = cfg.kb;
+    p.temperature = cfg.temperature;
+    p.mass = cfg.mass;
+    p.randomize_v = cfg.randomize_v;
+    p.randomize_state = cfg.randomize_state;
+    p.same_initial_barrier = cfg.same_initial_barrier;
+    p.periodic_length = cfg.periodic_length;
+    for (int b = 0; b < 2; ++b) {
+        for (int s = 0; s < 2; ++s) p.height[b][s] = cfg.height[b][s];
+        p.half_width[b] = cfg.half_width[b];
+        p.position[b] = cfg.position[b];
+    }
+    p.timestep = cfg.timestep;
+    p.nsteps = cfg.nsteps;
+    p.save_freq = cfg.save_freq;
+    p.tau = cfg.tau;
+    p.initial_state = cfg.initial_state;
+    p.equal_probability = cfg.equal_probability;
+    p.eq_constant = cfg.eq_constant;
+    p.b_waiting = cfg.waiting;
+    p.b_fpt = cfg.fpt;
+    p.b_tpt = cfg.tpt;
+    p.b_fptcg = cfg.fptcg;
+    p.n_cg = (int)ncg;
+    for (size_t j = 0; j < ncg && j < LDPCP_MAX_CG; ++j) p.cg_size[j] = cfg.cg_size[j];
+    p.save_all = cfg.write_x;
+    p.barrier_window = 4096;
+    p.tick_capacity = tick_cap;
+
+    std::vector<ldpcp_lane> lanes(W);
+    std::vector<double> msd0((size_t)W * nl);
+    std::vector<double> ticks(cfg.waiting ? (size_t)W * tick_cap : 0);
+    const uint64_t nf = cfg.nsteps / cfg.save_freq + 1;
+    std::vector<double> frames(cfg.write_x ? (size_t)7 * W * nf : 0);
+
+    for (int i = 0; i < cfg.ntrial; ++i) {
+        fprintf(stderr, "##################################################################\n");
+        fprintf(stderr, "Trial %10d/%10d\n", i, cfg.ntrial);
+        fprintf(stderr, "##################################################################\n");
+        fprintf(stderr, "   Integrate\n");
+        for (int r = 0; r < R; ++r) {
+            auto t0 = clock::now();
+            p.seed = (uint64_t)(uint32_t)seeds[r];
+            ldpcp_run* h = nullptr;
+            if (ldpcp_run_create(&p, (uint32_t)W, (uint32_t)((uint64_t)i * W), &h)) die("ldpcp_run_create");
+            if (ldpcp_run_integrate(h, cfg.steps_per_launch)) die("ldpcp_run_integrate");
+            auto t1 = clock::now();
+            t_integrate += std::chrono::duration<double>(t1 - t0).count();
+            if (ldpcp_run_lanes(h, 0, (uint32_t)W, lanes.data())) die("ldpcp_run_lanes");
+            if (ldpcp_run_msd(h, interval.data(), (uint32_t)nl, msd0.data())) die("ldpcp_run_msd");
+            if (cfg.waiting && ldpcp_run_ticks(h, 0, (uint32_t)W, ticks.data())) die("ldpcp_run_ticks");
+            if (cfg.write_x)
+                for (int field = 0; field < 7; ++field)
+                    if (ldpcp_run_frames(h, field, 0, (uint32_t)W, frames.data() + (size_t)field * W * nf))
+                        die("ldpcp_run_frames");
+            ldpcp_run_destroy(h);
+            if (r == 0) fprintf(stderr, "   Post Analysis\n");
+            // per-lane post analysis (Single.h:3452-3580)
+            for (int ch = 0; ch < W; ++ch) {
+                const ldpcp_lane& L = lanes[ch];
+                for (size_t t = 0; t < nl; ++t) {
+                    const double v = msd0[(size_t)ch * nl + t];
+                    msd[r][t] += v;
+                    if (isnan(v) || isinf(v)) fprintf(stderr, "Error: on node %10d, ill msd0\n", r);
+                }
+                if (cfg.waiting) {
+                    if (L.wt_ticks <= tick_cap) {
+                        const double* tk = ticks.data() + (size_t)ch * tick_cap;
+                        for (uint64_t t = 1; t < L.wt_ticks; ++t) wt_list[r].push_back((tk[t] - tk[t - 1]) * cfg.timestep);
+                    } else {
+                        wt_exact = false;
+                    }
+                    wt_sum[r] += L.wt_sum;
+                    wt_sum2[r] += L.wt_sum2;
+                    wt_n[r] += (double)L.wt_n;
+                    if (L.wt_n > 0) wt_max[r] = std::max(wt_max[r], L.wt_max);
+                }
+                if (cfg.v0_distribution) h_v0[r].add(L.v0_instance);
+                if (cfg.state0_distribution) h_s0[r].add(L.state0_instance);
+                if (cfg.write_x) {
+                    char buf[512];
+                    snprintf(buf, sizeof(buf), "%s_trajectory_%010d_%010d_%010d.log", cfg.output_base.c_str(), i, ch, r);
+                    FILE* f = open_output(buf);
+                    if (f) {
+                        hd.print(f);
+                        auto fr = [&](int field, uint64_t t) { return frames[((size_t)field * W + ch) * nf + t]; };
+                        for (uint64_t t = 0; t < nf; ++t)
+                            fprintf(f, " %23.16f %13.7f %13.7f %13.7f %13.7f %13.7f %13.7f %13.7f\n",
+                                    hd.save_dt * (double)t, fr(0, t), cfg.overdamped ? 0.0 : fr(1, t), fr(2, t),
+                                    fr(6, t), fr(3, t), cfg.overdamped ? 0.0 : fr(4, t), fr(5, t));
+                        if (cfg.waiting) {
+                            fprintf(f, "# event tick\n");
+                            const uint64_t n = std::min<uint64_t>(L.wt_ticks, tick_cap);
+                            for (uint64_t t = 0; t < n; ++t)
+                                fprintf(f, "# %10zd %23.16E\n", (size_t)t, ticks[(size_t)ch * tick_cap + t] * cfg.timestep);
+                        }
+                        fclose(f);
+                    }
+                }
+                if (cfg.fpt) {
+                    fpt[r].push_back(L.fpt_n ? L.fpt_sum / (double)L.fpt_n : 0.0);
+                    fpt2[r].push_back(L.fpt2_n ? L.fpt2_sum / (double)L.fpt2_n : 0.0);
+                }
+                if (cfg.tpt) {
+                    tpt[r].push_back(L.tpt_n ? L.tpt_sum / (double)L.tpt_n : 0.0);
+                    tpt2[r].push_back(L.tpt2_n ? L.tpt2_sum / (double)L.tpt2_n : 0.0);
+                }
+                for (size_t j = 0; j < ncg; ++j) cg[j][r].push_back(L.cg_n[j] ? L.cg_sum[j] / (double)L.cg_n[j] : 0.0);
+                state_count[r][0] += L.state_count[0];
+                state_count[r][1] += L.state_count[1];
+            }
+            t_analysis += std::chrono::duration<double>(clock::now() - t1).count();
+        }
+        fprintf(stderr, "   Done %10d/%10d\n", i + 1, cfg.ntrial);
+    }
+    const auto t_post0 = clock::now();
+    const std::string& ob = cfg.output_base;
+    auto combined = [&](const std::vector<std::vector<double>>& per_rank) {
+        std::vector<double> out;  // MPIKernel::CombineVectorDoubleToMaster: rank-major
+        for (const auto& v : per_rank) out.insert(out.end(), v.begin(), v.end());
+        return out;
+    };
+    if (cfg.fpt) {
+        const auto a = combined(fpt), b = combined(fpt2);
+        write_dat(ob + "_firstPassageTime.dat", a);
+        write_dat(ob + "_firstPassageTime2.dat", b);
+        h_init.normalize();
+        h_inst.normalize();
+        write_mean_log(ob + "_firstPassageTime.log", hd, a);
+        write_mean_log(ob + "_firstPassageTime2.log", hd, b);
+        for (int k = 0; k < 2; ++k) {
+            FILE* f = open_output(ob + (k ? "_instantBarrierHeightHistogram.log" : "_initialBarrierHeightHistogram.log"));
+            if (f) {
+                hd.print(f);
+                (k ? h_inst : h_init).print(f);
+                fclose(f);
+            }
+        }
+    }
+    {
+        double s0 = 0.0, s1 = 0.0;
+        for (int r = 0; r < R; ++r) {
+            s0 += state_count[r][0];
+            s1 += state_count[r][1];
+        }
+        FILE* f = open_output(ob + "_stateCount.log");
+        if (f) {
+            hd.print(f);
+            fprintf(f, "# state 0 %23.0f %13.7f\n", s0, s0 / (s0 + s1));
+            fprintf(f, "# state 1 %23.0f %13.7f\n", s1, s1 / (s0 + s1));
+            fclose(f);
+        }
+    }
+    if (cfg.tpt) {
+        const auto a = combined(tpt), b = combined(tpt2);
+        write_dat(ob + "_transitionPathTime.dat", a);
+        write_dat(ob + "_transitionPathTime2.dat", b);
+        write_mean_log(ob + "_transitionPathTime.log", hd, a);
+        write_mean_log(ob + "_transitionPathTime2.log", hd, b);
+    }
+    if (cfg.fptcg) {
+        std::vector<std::vector<double>> all(ncg);
+        for (size_t j = 0; j < ncg; ++j) {
+            all[j] = combined(cg[j]);
+            char buf[512];
+            snprintf(buf, sizeof(buf), "%s_firstPassageTimeCG2_%04d_%04d.dat", ob.c_str(), (int)j, cfg.cg_size[j]);
+            write_dat(buf, all[j]);
+        }
+        FILE* f = open_output(ob + "_firstPassageTimeCG2.log");
+        if (f) {
+            hd.print(f);
+            for (size_t j = 0; j < ncg; ++j) {
+                fprintf(f, "# %4d %4d\n", (int)j, cfg.cg_size[j]);
+                fprintf(f, "# average %23.16E\n", average(all[j]));
+                fprintf(f, "# sd      %23.16E\n", rmsd(all[j]));
+                fprintf(f, "# count   %23zd\n", all[j].size());
+            }
+            fclose(f);
+        }
+    }
+    auto sum_hist = [&](std::vector<Histogram>& hs) {
+        Histogram out = hs[0];  // MPIKernel::SumVectorDouble into the master, rank by rank
+        for (int r = 1; r < R; ++r)
+            for (int k = 0; k < out.size; ++k) out.count[k] += hs[r].count[k];
+        out.normalize();
+        return out;
+    };
+    if (cfg.v0_distribution) {
+        Histogram h = sum_hist(h_v0);
+        FILE* f = open_output(ob + "_V0Distribution.log");
+        if (f) {
+            hd.print(f);
+            h.print(f);
+            fclose(f);
+        }
+    }
+    if (cfg.state0_distribution) {
+        Histogram h = sum_hist(h_s0);
+        FILE* f = open_output(ob + "_State0Distribution.log");
+        if (f) {
+            hd.print(f);
+            h.print(f);
+            fclose(f);
+        }
+    }
+    {
+        std::vector<double> total = msd[0];
+        for (int r = 1; r < R; ++r)
+            for (size_t t = 0; t < nl; ++t) total[t] += msd[r][t];
+        FILE* f = open_output(ob + "_msd.log");
+        if (f) {
+            hd.print(f);
+            for (size_t t = 0; t < nl; ++t) {
+                const double v = total[t] / (double)hd.nrun;
+                if (cfg.model == 0) fprintf(f, "%14.7E %14.7E\n", hd.save_dt * (double)interval[t], v);
+                else fprintf(f, "%13.7f %13.7f\n", hd.save_dt * (double)interval[t], v);
+            }
+            fclose(f);
+        }
+    }
+    if (cfg.waiting) {
+        double sum = 0.0, sum2 = 0.0, n = 0.0, mx = 0.0;
+        for (int r = 0; r < R; ++r) {
+            if (wt_exact) {
+                // STATISTICS::SUM / SUM2 / MAX over the rank's list, then the all-reduce
+                double s = 0.0, s2 = 0.0, m = 0.0;
+                for (double w : wt_list[r]) {
+                    s += w;
+                    s2 = fma(w, w, s2);
+                }
+                if (!wt_list[r].empty()) {
+                    m = -1.7976931348623157e308;
+                    for (double w : wt_list[r]) m = std::max(w, m);
+                }
+                sum += s;
+                sum2 += s2;
+                n += (double)wt_list[r].size();
+                mx = std::max(mx, m);
+            } else {
+                sum += wt_sum[r];
+                sum2 += wt_sum2[r];
+                n += wt_n[r];
+                mx = std::max(mx, wt_max[r]);
+            }
+        }
+        const double avg = sum / n, avg2 = sum2 / n;
+        const double Rv = fma(-avg, avg, avg2) / avg / avg;
+        int nbin = 100;
+        if (n < 1000) nbin = std::max<int>(10, (int)n / 100);
+        Histogram h;
+        h.allocate(0.0, mx + 0.1, nbin);
+        h.normalize();
+        FILE* f = open_output(ob + "_waitingTime.log");
+        if (f) {
+            hd.print(f);
+            fprintf(f, "# R      = %13.7f\n", Rv);
+            fprintf(f, "# Nwt    = %13zd\n", (size_t)n);
+            h.print(f);
+            fclose(f);
+        }
+    }
+    const double t_post = std::chrono::duration<double>(clock::now() - t_post0).count();
+    const double t_total = std::chrono::duration<double>(clock::now() - t_start).count();
+    {
+        FILE* f = open_output(ob + "_time.log");
+        if (f) {
+            for (int r = 0; r < R; ++r) fprintf(f, "Rank %10d Seed %10d\n", r, seeds[r]);
+            fprintf(f, "Time consumed %s %s\n", time_readable(t_total).c_str(), time_readable(0.0).c_str());
+            fprintf(f, "Integrate     %s %s\n", time_readable(t_integrate).c_str(), time_readable(0.0).c_str());
+            fprintf(f, "Analysis      %s %s\n", time_readable(t_analysis).c_str(), time_readable(0.0).c_str());
+            fprintf(f, "Post          %s %s\n", time_readable(t_post).c_str(), time_readable(0.0).c_str());
+            fclose(f);
+        }
+    }
+    return 0;
+}
+
+}  // namespace ldpcp_host

@@ -50,6 +50,7 @@ class LangevinEquation:
         self.m_FirstPassageTimeCGSize = []
         self.m_SaveAll = False
         self.barrier_window = 1024
+        self.tick_capacity = 0
         self.steps_per_launch = 1 << 20
         self._h = None
         self._seed = 0
@@ -90,6 +91,7 @@ class LangevinEquation:
         p.save_all = int(self.m_SaveAll)
         p.seed = self._seed & 0xFFFFFFFFFFFFFFFF
         p.barrier_window = self.barrier_window
+        p.tick_capacity = self.tick_capacity
         return p
 
     def Initialize(self, seed):
@@ -156,6 +158,15 @@ class LangevinEquation:
                     "ldpcp_run_frames")
             self._frames[field] = out
         return self._frames[field]
+
+    def ticks(self):
+        """Barrier ticks per trajectory (GetBarrierTick), truncated to tick_capacity."""
+        cap = self.tick_capacity
+        out = np.zeros((self.n_traj, cap))
+        N.check(N.load().ldpcp_run_ticks(self._h, 0, self.n_traj, out.ctypes.data_as(C.POINTER(C.c_double))),
+                "ldpcp_run_ticks")
+        lanes = self.lanes()
+        return [out[i, :min(cap, lanes[i].wt_ticks)] for i in range(self.n_traj)]
 
     def msd(self, lags):
         lags = np.ascontiguousarray(lags, np.uint64)
