@@ -103,12 +103,88 @@ __device__ __forceinline__ void box_muller_f32(uint32_t a, uint32_t b, float& g0
 {
     const float u1 = unit_f32(a);
     const float u2 = unit_f32(b);
-    const float r = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+    const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
     g0 = r * __builtin_amdgcn_cosf(u2);
     g1 = r * __builtin_amdgcn_sinf(u2);
 }
 
 __device__ __forceinline__ bool below(uint32_t w, uint64_t T) { return (uint64_t)w < T; }
+
+// Wave-uniform copy of a set: the fields the step loop touches every step are
+// moved to scalar registers, so Philox's round keys and the integrator's
+// constants cost no VGPRs and their updates run on the scalar unit.
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ uint64_t uni(uint64_t v) { return ((uint64_t)uni((uint32_t)(v >> 32)) << 32) | uni((uint32_t)v); }
+__device__ __forceinline__ double uni(double v) { return __longlong_as_double((long long)uni((uint64_t)__double_as_longlong(v))); }
+
+template <bool F32, int MODEL, bool OVER>
+__device__ __forceinline__ SetDev uniform_copy(const SetDev& src)
+{
+    SetDev s = src;
+    s.target = uni(s.target);
+    s.equal_prob = uni(s.equal_prob);
+    s.tau_zero = uni(s.tau_zero);
+    s.key0 = uni(s.key0);
+    s.key1 = uni(s.key1);
+    s.slot_base = uni(s.slot_base);
+    s.max_steps = uni(s.max_steps);
+    s.T_flip = uni(s.T_flip);
+    if (F32) {
+        s.fL = uni(s.fL);
+        s.fhalfL = uni(s.fhalfL);
+        if (OVER) {
+            s.fdtgm = uni(s.fdtgm);
+        } else {
+            s.fminv_halfdt = uni(s.fminv_halfdt);
+            s.fhalfdt = uni(s.fhalfdt);
+            s.fexpmgdt = uni(s.fexpmgdt);
+        }
+        s.fvar = uni(s.fvar);
+        if (MODEL == 0) {
+            s.fw = uni(s.fw);
+            s.fS[0] = uni(s.fS[0]);
+            s.fS[1] = uni(s.fS[1]);
+        } else {
+            s.fxc = uni(s.fxc);
+            for (int b = 0; b < 2; ++b) {
+                s.fxb[b] = uni(s.fxb[b]);
+                s.flo[b] = uni(s.flo[b]);
+                s.fhi[b] = uni(s.fhi[b]);
+                for (int k = 0; k < 2; ++k)
+                    for (int side = 0; side < 2; ++side) s.fslope[b][k][side] = uni(s.fslope[b][k][side]);
+            }
+        }
+    } else {
+        s.L = uni(s.L);
+        s.Linv = uni(s.Linv);
+        if (OVER) {
+            s.dtgm = uni(s.dtgm);
+        } else {
+            s.minv = uni(s.minv);
+            s.halfdt = uni(s.halfdt);
+            s.expmgdt = uni(s.expmgdt);
+        }
+        s.var = uni(s.var);
+        if (MODEL == 0) {
+            s.w = uni(s.w);
+            s.S[0] = uni(s.S[0]);
+            s.S[1] = uni(s.S[1]);
+        } else {
+            s.xc = uni(s.xc);
+            for (int b = 0; b < 2; ++b) {
+                s.xb[b] = uni(s.xb[b]);
+                s.lo[b] = uni(s.lo[b]);
+                s.hi[b] = uni(s.hi[b]);
+            }
+        }
+    }
+    return s;
+}
 
 __device__ __forceinline__ void record_absorption(const SetDev& S, const SweepPtrs& P, int set, uint32_t gi,
                                                   uint32_t step, uint32_t tpt0)
@@ -122,7 +198,8 @@ __device__ __forceinline__ void record_absorption(const SetDev& S, const SweepPt
     atomicAdd(P.tpt_sum + set, tpt);
     atomicAdd(P.tpt_sum2 + set, tpt * tpt);
     atomicAdd(P.n_absorbed + set, 1ull);
-    int bin = (int)floor((log10(fpt) - S.hist_lo) * S.hist_inv);
+    // bin on log10(fpt) = log2(fpt) * log10(2), in FP32 (only the bin index is kept)
+    int bin = (int)((__builtin_amdgcn_logf((float)fpt) * 0.30102999566f - (float)S.hist_lo) * (float)S.hist_inv);
     bin = bin < 0 ? 0 : (bin >= LDPCP_HIST_BINS ? LDPCP_HIST_BINS - 1 : bin);
     atomicAdd(P.hist + (uint64_t)set * LDPCP_HIST_BINS + bin, 1ull);
 }
@@ -272,19 +349,15 @@ __device__ __forceinline__ float force_f32(const SetDev& S, float xp, int state)
     }
 }
 
-template <int MODEL, bool OVER>
-__global__ void __launch_bounds__(kBlock) sweep_step_f32(SweepPtrs P, const uint32_t* __restrict__ live,
-                                                         uint8_t* __restrict__ alive_out, uint32_t n_live,
-                                                         uint32_t s0, uint32_t K, int sample_lag)
+// One trajectory for K steps.  On set-uniform waves S is a copy whose hot
+// fields were moved to scalar registers (uniform_copy); mixed waves read the
+// set through a reference.
+template <int MODEL, bool OVER, typename SetT>
+__device__ __forceinline__ void step_f32_body(const SweepPtrs& P, SetT S, int set, uint32_t gi, uint32_t li,
+                                              uint8_t* __restrict__ alive_out, uint32_t s0, uint32_t K,
+                                              int sample_lag, bool& active, double& msd_v)
 {
-    const uint32_t li = blockIdx.x * kBlock + threadIdx.x;
-    const bool in = li < n_live;
-    const uint32_t gi = in ? live[li] : 0u;
-    const int set = in ? P.set_of[gi] : -1;
-    bool active = in;
-    double msd_v = 0.0;
-    if (in) {
-        const SetDev& S = P.sets[set];
+    {
         const uint32_t tid = (uint32_t)(gi - S.first);
         float xp = P.xp[gi], comp = P.comp[gi], v = P.fv[gi], R = P.fR[gi];
         int cell = P.cell[gi];
@@ -400,6 +473,34 @@ __global__ void __launch_bounds__(kBlock) sweep_step_f32(SweepPtrs P, const uint
             msd_v = (x - S.x0) * (x - S.x0);
         }
     }
+}
+
+template <int MODEL, bool OVER>
+__global__ void __launch_bounds__(kBlock) sweep_step_f32(SweepPtrs P, const SetDev* __restrict__ sets,
+                                                         const uint32_t* __restrict__ live,
+                                                         uint8_t* __restrict__ alive_out, uint32_t n_live,
+                                                         uint32_t s0, uint32_t K, int sample_lag)
+{
+    const uint32_t li = blockIdx.x * kBlock + threadIdx.x;
+    const bool in = li < n_live;
+    const uint32_t gi = in ? live[li] : 0u;
+    const int set = in ? P.set_of[gi] : -1;
+    // lane 0 of a wave is live whenever any lane is (the live list is dense)
+    // lane 0 of a wave is live whenever any lane is (the live list is dense)
+    const int set0 = __builtin_amdgcn_readfirstlane(set);
+    const bool uniform = __ballot(in && set != set0) == 0;
+    bool active = false;
+    double msd_v = 0.0;
+    if (in) {
+        // set-uniform waves (all but the few at set boundaries) keep the set's
+        // constants in scalar registers
+        if (uniform)
+            step_f32_body<MODEL, OVER, const SetDev>(P, uniform_copy<true, MODEL, OVER>(sets[set0]), set, gi, li,
+                                                      alive_out, s0, K, sample_lag, active, msd_v);
+        else
+            step_f32_body<MODEL, OVER, const SetDev&>(P, sets[set], set, gi, li, alive_out, s0, K, sample_lag, active,
+                                                       msd_v);
+    }
     if (sample_lag >= 0) wave_add_by_set(set, active, msd_v, P.msd_sum, sample_lag, P.msd_n);
 }
 
@@ -425,19 +526,12 @@ __device__ __forceinline__ double force_f64(const SetDev& S, double x, double pe
     }
 }
 
-template <int MODEL, bool OVER>
-__global__ void __launch_bounds__(kBlock) sweep_step_f64(SweepPtrs P, const uint32_t* __restrict__ live,
-                                                         uint8_t* __restrict__ alive_out, uint32_t n_live,
-                                                         uint32_t s0, uint32_t K, int sample_lag)
+template <int MODEL, bool OVER, typename SetT>
+__device__ __forceinline__ void step_f64_body(const SweepPtrs& P, SetT S, int set, uint32_t gi, uint32_t li,
+                                              uint8_t* __restrict__ alive_out, uint32_t s0, uint32_t K,
+                                              int sample_lag, bool& active, double& msd_v)
 {
-    const uint32_t li = blockIdx.x * kBlock + threadIdx.x;
-    const bool in = li < n_live;
-    const uint32_t gi = in ? live[li] : 0u;
-    const int set = in ? P.set_of[gi] : -1;
-    bool active = in;
-    double msd_v = 0.0;
-    if (in) {
-        const SetDev& S = P.sets[set];
+    {
         const uint32_t tid = (uint32_t)(gi - S.first);
         double x = P.x[gi], v = P.v[gi], R = P.R[gi];
         uint32_t flags = P.flags[gi];
@@ -536,6 +630,33 @@ __global__ void __launch_bounds__(kBlock) sweep_step_f64(SweepPtrs P, const uint
         alive_out[li] = (flags & F_ALIVE) ? 1 : 0;
         active = (sample_lag >= 0) && !(done && last != s_end) && (s_end == s0 + K);
         if (active) msd_v = (x - S.x0) * (x - S.x0);
+    }
+}
+
+template <int MODEL, bool OVER>
+__global__ void __launch_bounds__(kBlock) sweep_step_f64(SweepPtrs P, const SetDev* __restrict__ sets,
+                                                         const uint32_t* __restrict__ live,
+                                                         uint8_t* __restrict__ alive_out, uint32_t n_live,
+                                                         uint32_t s0, uint32_t K, int sample_lag)
+{
+    const uint32_t li = blockIdx.x * kBlock + threadIdx.x;
+    const bool in = li < n_live;
+    const uint32_t gi = in ? live[li] : 0u;
+    const int set = in ? P.set_of[gi] : -1;
+    // lane 0 of a wave is live whenever any lane is (the live list is dense)
+    const int set0 = __builtin_amdgcn_readfirstlane(set);
+    const bool uniform = __ballot(in && set != set0) == 0;
+    bool active = false;
+    double msd_v = 0.0;
+    if (in) {
+        // set-uniform waves (all but the few at set boundaries) keep the set's
+        // constants in scalar registers
+        if (uniform)
+            step_f64_body<MODEL, OVER, const SetDev>(P, uniform_copy<false, MODEL, OVER>(sets[set0]), set, gi, li,
+                                                      alive_out, s0, K, sample_lag, active, msd_v);
+        else
+            step_f64_body<MODEL, OVER, const SetDev&>(P, sets[set], set, gi, li, alive_out, s0, K, sample_lag, active,
+                                                       msd_v);
     }
     if (sample_lag >= 0) wave_add_by_set(set, active, msd_v, P.msd_sum, sample_lag, P.msd_n);
 }
@@ -874,10 +995,10 @@ static void launch_kind(ldpcp_sweep* h, int kind, uint32_t K, int lag)
     const uint32_t n = h->n_live[kind];
     const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
     if (h->precision == 32)
-        hipLaunchKernelGGL((sweep_step_f32<MODEL, OVER>), grid, block, 0, h->stream, h->P, h->live[kind],
+        hipLaunchKernelGGL((sweep_step_f32<MODEL, OVER>), grid, block, 0, h->stream, h->P, h->P.sets, h->live[kind],
                            h->P.alive_flag + h->kind_first[kind], n, h->step, K, lag);
     else
-        hipLaunchKernelGGL((sweep_step_f64<MODEL, OVER>), grid, block, 0, h->stream, h->P, h->live[kind],
+        hipLaunchKernelGGL((sweep_step_f64<MODEL, OVER>), grid, block, 0, h->stream, h->P, h->P.sets, h->live[kind],
                            h->P.alive_flag + h->kind_first[kind], n, h->step, K, lag);
 }
 

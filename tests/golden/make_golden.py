@@ -12,6 +12,8 @@ Outputs (tests/golden/):
   quick_test_frames.npz   x frames 0..10000 of the 16 lanes, [rank*4+lane][frame]
   quick_test_state.json   per-lane end state, event counts and means, RNG
                           progress, barrier tables, output-file values
+  quick_test_headers.json text of some recorded output files (layout checks)
+  quick_test.inp          the quick test's input file
 """
 import json
 import os
@@ -97,6 +99,18 @@ def main():
              "ranks": ranks, "lanes": lanes, "outputs": outputs}
     with open(os.path.join(HERE, "quick_test_state.json"), "w") as f:
         json.dump(state, f, separators=(",", ":"))
+    # text of a few output files (headers and bodies), for layout checks
+    headers = {}
+    for name in ("firstPassageTime.log", "msd.log", "stateCount.log", "waitingTime.log", "State0Distribution.log"):
+        with open(os.path.join(JOB, f"{BASE}_{name}")) as f:
+            headers[name] = f.read().splitlines()[:200]
+    with open(os.path.join(HERE, "quick_test_headers.json"), "w") as f:
+        json.dump(headers, f, indent=0)
+    # the quick test's input file itself
+    with open("/root/reference/sample_input/quick_test/2.0_7.2_m1.0_1.00e-05.inp") as f:
+        inp = f.read()
+    with open(os.path.join(HERE, "quick_test.inp"), "w") as f:
+        f.write(inp)
     print("frames", frames.shape, "lanes", len(lanes))
 
 

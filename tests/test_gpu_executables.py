@@ -1,0 +1,170 @@
+"""End-to-end runs of the drop-in executables on the GPU.
+
+The expected output values are recomputed with the CPU oracle (lane mode,
+same Philox keys: rank r's glibc seed, trajectory trial*W + lane) and the
+runner's reductions (rank-major per-lane means, MSD summed lane by lane,
+state counts); file layouts are checked against the reference's recorded
+quick-test outputs.
+"""
+import ctypes as C
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(REPO, "langevindynamicsforpcp_amd", "bin")
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+SINGLE_INP = """\
+OUTPUTBASE           out
+OVERDAMPED           {over}
+X0                   -1.05
+V0                   1.0
+GAMMA                1.0
+KB                   1.0
+TEMPERATURE          1.0
+MASS                 1.0
+RANDOMIZEV           true
+PERIODICLENGTH       2.1
+BARRIERHEIGHTS       3.0 1.0
+BARRIERHALFWIDTH     0.6
+TIMESTEP             0.001
+NUMBEROFSTEPS        200000
+NUMBEROFPREVIOUSSTEPS 0
+NUMBEROFRUNS         2
+FREQSAVE             100
+FREQSTAT             1
+DISTRIBUTIONV0       true
+DISTRIBUTIONS0       true
+TAU                  0.001
+INITIALSTATE         0
+RANDOMIZESTATES      true
+UNIFORMINITALSTATES  false
+EQ                   true 1.0
+WAITINGTIME          true
+FPT                  true
+TPT                  true
+FPTCG                true 2
+FPTCG                true 3
+FPTCG                true 10
+READRESTART          false
+WRITERESTART         false
+INPUTRESTART         none
+OUTPUTRESTART        none
+WRITEX               true
+NUMBEROFLOGSAMPLES   20
+RANDOMSEED           123
+PRINTTIMER           false
+"""
+
+
+def rank_seeds(master, ranks):
+    libc = C.CDLL("libc.so.6")
+    libc.srand(master)
+    out = []
+    while len(out) < ranks:
+        v = libc.rand() % 2147483647
+        if v not in out:
+            out.append(v)
+    return out
+
+
+def oracle_lanes(oracle, p, seed, trials, W):
+    res = []
+    nfr = p.nsteps // p.save_freq + 1
+    for i in range(trials):
+        for ch in range(W):
+            out = ol.Lane()
+            fr = np.zeros(nfr)
+            oracle.ldo_lane_run_philox(C.byref(p), C.byref(out), seed, i * W + ch, 0, 0, ol.dptr(fr), nfr)
+            res.append((out, fr))
+    return res
+
+
+@pytest.mark.parametrize("over", ["true", "false"])
+def test_single_barrier_executable_matches_oracle(oracle, tmp_path, over):
+    (tmp_path / "run.inp").write_text(SINGLE_INP.format(over=over))
+    ranks, W, trials = 3, 4, 2
+    print(f"[gpu exe] single over={over}", flush=True)
+    r = subprocess.run([os.path.join(BIN, "langevinDynamicsSingleBarrier"), "-i", "run.inp", "--ranks", str(ranks),
+                        "--simd-width", str(W)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    p = ol.make_params(model=0, overdamped=1 if over == "true" else 0, x0=-1.05, periodic_length=2.1,
+                       heights=[[3.0, 1.0], [3.0, 1.0]], half_width=[0.6, 0.6], timestep=1e-3, tau=1e-3,
+                       nsteps=200000, save_freq=100, randomize_v=1, cg=[2, 3, 10])
+    seeds = rank_seeds(123, ranks)
+    lanes = []
+    for rk in range(ranks):
+        lanes += oracle_lanes(oracle, p, seeds[rk], trials, W)
+    mean = lambda s, n: s / n if n else 0.0  # noqa: E731
+    exp = {
+        "firstPassageTime.dat": [mean(L.fpt_sum, L.fpt_n) for L, _ in lanes],
+        "firstPassageTime2.dat": [mean(L.fpt2_sum, L.fpt2_n) for L, _ in lanes],
+        "transitionPathTime.dat": [mean(L.tpt_sum, L.tpt_n) for L, _ in lanes],
+        "transitionPathTime2.dat": [mean(L.tpt2_sum, L.tpt2_n) for L, _ in lanes],
+        "firstPassageTimeCG2_0000_0002.dat": [mean(L.cg_sum[0], L.cg_n[0]) for L, _ in lanes],
+        "firstPassageTimeCG2_0002_0010.dat": [mean(L.cg_sum[2], L.cg_n[2]) for L, _ in lanes],
+    }
+    for name, want in exp.items():
+        got = np.fromfile(tmp_path / f"out_{name}", "<f8")
+        np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12, err_msg=name)
+    # msd.log: lags from GenerateLog10Scale, per-lane time-origin MSD summed rank-major / nrun
+    lags = np.zeros(512, np.uint64)
+    nl = oracle.ldo_log10_scale(0.1, 2001, 1000, 20, 1, lags.ctypes.data_as(C.POINTER(C.c_uint64)), 512)
+    lags = lags[:nl]
+    total = np.zeros(nl)
+    out = np.zeros(nl)
+    for _, fr in lanes:
+        oracle.ldo_msd_brute(ol.dptr(fr), len(fr), lags.ctypes.data_as(C.POINTER(C.c_uint64)), nl, ol.dptr(out))
+        total += out
+    msd = np.loadtxt(tmp_path / "out_msd.log", comments="#")
+    np.testing.assert_allclose(msd[:, 0], 0.1 * lags, rtol=1e-7)
+    np.testing.assert_allclose(msd[:, 1], total / len(lanes), rtol=1e-6)
+    # state counts and initial-state histogram are exact
+    s0 = sum(L.state_count[0] for L, _ in lanes)
+    s1 = sum(L.state_count[1] for L, _ in lanes)
+    sc = [ln for ln in (tmp_path / "out_stateCount.log").read_text().splitlines() if ln.startswith("# state")]
+    assert sc[0].split()[3] == f"{s0:.0f}" and sc[1].split()[3] == f"{s1:.0f}"
+    st0 = np.loadtxt(tmp_path / "out_State0Distribution.log", comments="#")
+    assert st0[:, 3].tolist() == [sum(1 for L, _ in lanes if L.state0_instance == 0.0),
+                                  sum(1 for L, _ in lanes if L.state0_instance == 1.0)]
+    # one trajectory log per (trial, lane, rank), frames first, then the event ticks
+    logs = sorted(f for f in os.listdir(tmp_path) if "_trajectory_" in f)
+    assert len(logs) == ranks * trials * W
+    body = np.loadtxt(tmp_path / logs[0], comments="#")
+    assert body.shape == (2001, 8)
+    np.testing.assert_allclose(body[:, 1], lanes[0][1], rtol=1e-6, atol=1e-6)
+    # waiting-time moments
+    wt = [ln for ln in (tmp_path / "out_waitingTime.log").read_text().splitlines() if ln.startswith("# Nwt")]
+    assert int(wt[0].split()[-1]) == sum(L.wt_n for L, _ in lanes)
+
+
+def test_header_layout_matches_reference(tmp_path):
+    """The PrintParameter block of our .log files has the reference's lines in
+    the reference's order (values differ only where the run differs)."""
+    with open(os.path.join(GOLD, "quick_test_headers.json")) as f:
+        ref = json.load(f)["firstPassageTime.log"]
+    inp = open(os.path.join(GOLD, "quick_test.inp")).read()
+    inp = inp.replace("3000000000", "3000000").replace("FREQSAVE             300000", "FREQSAVE             3000")
+    inp = inp.replace("WRITEX               true", "WRITEX               false")
+    inp = inp.replace("WRITERESTART         true", "WRITERESTART         false")
+    (tmp_path / "q.inp").write_text(inp)
+    r = subprocess.run([os.path.join(BIN, "langevinDynamicsSingleBarrier"), "-i", "q.inp", "--ranks", "4",
+                        "--simd-width", "4"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = (tmp_path / "m1.0_t1.00e-05_0_firstPassageTime.log").read_text().splitlines()
+    varying = ("Save frequency", "Save time step", "Number of step ", "Simulation time", "Number of traj",
+               "Memory usage", "# average", "# sd ", "Write trajectories", "Write restart")
+    assert len(got) == len(ref)
+    for a, b in zip(got, ref):
+        if any(v in b for v in varying):
+            assert a.split("=")[0] == b.split("=")[0]
+        else:
+            assert a == b
