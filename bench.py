@@ -98,6 +98,73 @@ def cpu_baseline(seconds=12.0):
                       f"on the same workload parameters, trajectories absorbed or capped at {steps} steps"}
 
 
+def pmc_child(args):
+    """Counter-run body: a short run of the timed workload, then a plain copy of
+    known size that calibrates the counters."""
+    sw, _ = time_engine(32, args.n_traj, 3, 1, args.steps_per_launch)
+    sw.close()
+    import torch
+    n = COPY_ELEMS
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+
+
+COPY_ELEMS = 1 << 26  # 256 MiB read + 256 MiB written per copy launch
+
+
+def read_counter_csv(out_dir, counter):
+    """{kernel name: [per-dispatch value in dispatch order]} from a rocprofv3 csv."""
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(out_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") == counter:
+                    rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    out = {}
+    for _, name, v in rows:
+        out.setdefault(name, []).append(v)
+    return out
+
+
+def pmc_traffic(args, timeout=240):
+    """HBM bytes per launch of the FP32 step kernel from two rocprofv3 --pmc runs
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
+    counter reads for a plain copy of known size.  None if rocprofv3 is missing
+    or a run fails."""
+    import shutil
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="ldpcp_pmc_")
+        cmd = [prof, "--pmc", counter, "--kernel-include-regex", "sweep_step|elementwise", "-d", d, "-o", "pmc",
+               "-f", "csv", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
+               "--n-traj", str(args.n_traj), "--steps-per-launch", str(args.steps_per_launch)]
+        try:
+            subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=timeout, check=False)
+        except subprocess.TimeoutExpired:
+            return None
+        vals = read_counter_csv(d, counter)
+        shutil.rmtree(d, ignore_errors=True)
+        step = [v for k, vs in vals.items() if "sweep_step" in k for v in vs]
+        copy = [v for k, vs in vals.items() if "elementwise" in k for v in vs]
+        if len(step) < 3 or not copy:
+            return None
+        known_kib = COPY_ELEMS * 4 / 1024.0
+        factor = known_kib / (sum(copy) / len(copy))
+        res[counter] = {"kib_per_launch": sum(step[-3:]) / 3, "copy_factor": factor,
+                        "bytes_per_launch": sum(step[-3:]) / 3 * 1024.0 * factor}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,7 +174,12 @@ def main():
     ap.add_argument("--n-traj", type=int, default=1 << 20)
     ap.add_argument("--no-finish", action="store_true", help="skip the run to completion (MFPT check)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; "
@@ -141,6 +213,7 @@ def main():
     sw32.close()
     sw64.close()
 
+    traffic = None if args.no_traffic else pmc_traffic(args)
     ops = OPS_PER_STEP["single_over"]
     achieved = k32 * ops / 1e12
     peak = P_ISSUE / 1e12
@@ -166,7 +239,12 @@ def main():
         "fp32_over_fp64": v32 / v64,
         "kernel_traj_steps_per_s": {"f32": k32, "f64": k64},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "Tops/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": (traffic["FETCH_SIZE"]["bytes_per_launch"] + traffic["WRITE_SIZE"]["bytes_per_launch"])
+                     if traffic else None,
+                     "traffic_detail": traffic,
+                     "avg_launch_us": 1e3 * r32["kernel_ms"] / r32["launches"],
+                     "traj_steps_per_launch": r32["traj_steps"] / r32["launches"],
                      "ops_per_traj_step": ops,
                      "note": "VALU issue bound (no MFMA, HBM traffic per step ~0); peak = 256 CU x 64 lanes x "
                              "2.4 GHz non-packed issue; frac vs the 157.3 TFLOP/s packed-FP32 vector peak is "
