@@ -1,20 +1,28 @@
-"""Throughput of the Langevin hot path on one MI355X.
+"""Throughput of the Langevin first-passage hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--steps-per-launch S]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--serve-seconds S]
 
-Workload (the single-GPU configuration the metric is quoted on): single-barrier
-model, overdamped, barrier alternating between 7.2 and 2.0 (Eb = 2.0,
-amplitude 3.6 Eb, sample_input/CuIDB_single geometry: L = 5.8, w = 0.65,
-dt = 1e-4, tau = 1e-5), 2^20 trajectories starting at the well centre
-X0 = -2.9, Philox seed 1000, step cap 1e8, absorbed trajectories retired after
-every launch.  A bench "step" is one launch of S integration steps over every
-live trajectory.  The FP32+compensated engine is timed (value) next to the plain
-FP64 engine; afterwards both run to completion and their mean first-passage
+Workload: the single-barrier model with 2^20 trajectories per parameter set,
+Eb in {2, 3, 4, 5} with the barrier alternating between 3.6 Eb and Eb
+(sample_input/CuIDB_single geometry: L = 5.8, w = 0.65, dt = 1e-4,
+tau = 1e-5), underdamped and overdamped -- 8 sets, 2^23 trajectories in one
+ragged batch -- starting at rest in the well centre X0 = -2.9, Philox seed
+1000 + Eb index, step cap 1e8, absorbed trajectories retired after every
+launch.  A bench "step" is one launch (steps_per_launch integration steps of
+every live trajectory).  The FP32+compensated engine is timed (value) next to
+the plain FP64 engine, both with the autotuned launch shape.  Afterwards the
+Eb = 2 sets run to completion in both arithmetics and their mean first-passage
 times are compared.
+
+Serving: a Poisson stream of sweep requests (20 per second for --serve-seconds,
+seed 11; single or double barrier, under- or overdamped, Eb ~ U[2, 6.5] with
+ratio 3.6, 1e3 to 1e6 trajectories log-uniform, dt = 1e-3, tau = 1e-4, cap
+5e4 steps) into the FP32 request server with a 200 GB HBM budget; p50 / p95 /
+p99 request latency and the aggregate trajectory-steps/s are reported next to
+the plain FP64 engine running the same requests one at a time.
 """
 import argparse
 import json
-import math
 import os
 import subprocess
 import sys
@@ -25,63 +33,119 @@ sys.path.insert(0, REPO)
 
 # VALU lane-op issue rate of MI355X: 256 CU x 64 lanes/cycle x 2.4 GHz
 P_ISSUE = 256 * 64 * 2.4e9
-# algorithmic VALU ops per trajectory-step of the overdamped single-barrier
-# step: Philox4x32-10 (40) + Box-Muller (12) + flip (2) + integrate, indices,
-# telegraph, parameters, force and events (39)
-OPS_PER_STEP = {"single_over": 93, "single_under": 102, "double_over": 115, "double_under": 124}
+# algorithmic VALU ops per trajectory-step (Philox4x32-10 40, Box-Muller 12,
+# flip 2, and the integrator, indices, telegraph, force and events)
+OPS_PER_STEP = {(0, True): 93, (0, False): 102, (1, True): 115, (1, False): 124}
+EBS = (2.0, 3.0, 4.0, 5.0)
+STEPS_PER_LAUNCH = 4096
+COPY_ELEMS = 1 << 26  # 256 MiB read + 256 MiB written per calibration copy
 
 
-def workload_set(n_traj, seed=1000):
+def workload_sets(n_traj, ebs=EBS):
     from langevindynamicsforpcp_amd.sweep import sweep_set
-    return sweep_set(model=0, overdamped=True, eb=2.0, ratio=3.6, n_traj=n_traj, max_steps=10 ** 8, seed=seed,
-                     x0=-2.9, periodic_length=5.8, half_width=(0.65, 0.65), timestep=1e-4, tau=1e-5)
+    out = []
+    for over in (True, False):
+        for k, eb in enumerate(ebs):
+            out.append(sweep_set(model=0, overdamped=over, eb=eb, ratio=3.6, n_traj=n_traj, max_steps=10 ** 8,
+                                 seed=1000 + EBS.index(eb), x0=-2.9, periodic_length=5.8, half_width=(0.65, 0.65),
+                                 timestep=1e-4, tau=1e-5, randomize_v=False, v0=0.0))
+    return out
 
 
-def time_engine(precision, n_traj, steps, warmup, spl):
+def total_steps(sw, n):
+    return sum(sw.stats(i).traj_steps for i in range(n))
+
+
+def time_engine(precision, n_traj, steps, warmup, spl, tune=True):
     from langevindynamicsforpcp_amd.sweep import Sweep
-    sw = Sweep([workload_set(n_traj)], precision=precision)
-    # the first launches end on the dense early MSD lag steps (1, 2, 3, ...);
-    # run through that prefix before the warmup proper
-    steps_done = 0
-    while steps_done < 2 * spl:
-        sw.launch(spl)
-        steps_done = sw.stats(0).traj_steps // n_traj
+    sets = workload_sets(n_traj)
+    sw = Sweep(sets, precision=precision)
+    cfg = sw.tune() if tune else None
+    if spl:
+        c = sw.config()
+        sw.set_config(block=c.block, blocks_per_cu=c.blocks_per_cu, steps_per_launch=spl, use_graph=c.use_graph)
     for _ in range(warmup):
-        sw.launch(spl)
-    st0 = sw.stats(0).traj_steps
+        sw.launch()
+    per_set0 = [sw.stats(i).traj_steps for i in range(len(sets))]
     ms0, n0 = sw.kernel_time()
     t0 = time.perf_counter()
     for _ in range(steps):
-        sw.launch(spl)
+        sw.launch()
     wall = time.perf_counter() - t0
-    st1 = sw.stats(0).traj_steps
+    per_set1 = [sw.stats(i).traj_steps for i in range(len(sets))]
     ms1, n1 = sw.kernel_time()
-    res = {"traj_steps": st1 - st0, "wall_s": wall, "kernel_ms": ms1 - ms0, "launches": n1 - n0}
-    return sw, res
+    d = [b - a for a, b in zip(per_set0, per_set1)]
+    ops = sum(di * OPS_PER_STEP[(s.model, bool(s.overdamped))] for di, s in zip(d, sets)) / max(sum(d), 1)
+    c = sw.config()
+    res = {"traj_steps": sum(d), "wall_s": wall, "kernel_ms": ms1 - ms0, "launches": n1 - n0, "ops_per_step": ops,
+           "config": {"block": c.block, "blocks_per_cu": c.blocks_per_cu, "steps_per_launch": c.steps_per_launch,
+                      "use_graph": c.use_graph, "tuned": cfg is not None}}
+    sw.close()
+    return res
 
 
-def finish(sw, spl):
+def mfpt_check(precision, n_traj):
+    """Eb = 2 sets (overdamped, underdamped) run to completion."""
+    from langevindynamicsforpcp_amd.sweep import Sweep
+    sets = workload_sets(n_traj, ebs=(2.0,))
+    sw = Sweep(sets, precision=precision)
+    sw.tune()
     t0 = time.perf_counter()
-    sw.run(spl)
-    m, e = sw.mfpt(0)
-    st = sw.stats(0)
-    return {"mfpt": m, "mfpt_stderr": e, "n_absorbed": st.n_absorbed, "n_capped": st.n_capped,
-            "tpt_mean": st.tpt_sum / max(st.n_absorbed, 1), "seconds_to_finish": time.perf_counter() - t0}
+    sw.run()
+    out = []
+    for i, s in enumerate(sets):
+        m, e = sw.mfpt(i)
+        st = sw.stats(i)
+        out.append({"overdamped": bool(s.overdamped), "mfpt": m, "stderr": e, "n_absorbed": st.n_absorbed,
+                    "n_capped": st.n_capped, "tpt_mean": st.tpt_sum / max(st.n_absorbed, 1)})
+    sw.close()
+    return out, time.perf_counter() - t0
+
+
+def serving(seconds, budget_gb, rate=20.0, seed=11, fp64_limit_s=60.0):
+    import numpy as np
+    from langevindynamicsforpcp_amd import server as S
+    reqs = S.poisson_requests(rate, seconds, seed)
+    srv = S.Server(budget_bytes=int(budget_gb * 1e9), precision=32)
+    # warm the graphs and kernels on a throwaway request before the clock starts
+    srv.submit(reqs[0][1])
+    while not srv.completed():
+        srv.pump(1, wait=True)
+    res = S.serve(srv, reqs)
+    info = srv.info()
+    srv.close()
+    steps = sum(r.traj_steps for r in res["results"])
+    out = {"requests": len(reqs), "rate_per_s": rate, "duration_s": seconds, "seed": seed,
+           "budget_gb": budget_gb, "wall_s": res["wall"], "traj_steps": steps,
+           "traj_steps_per_s": steps / res["wall"], "latency_s": S.percentiles(res["latency"]),
+           "latency_mean_s": float(np.mean(res["latency"])), "left_resident": info["resident"]}
+    # the plain FP64 engine, one job at a time, on the requests its time limit allows
+    S.one_at_a_time(reqs[:1], precision=64)  # warm-up
+    one = S.one_at_a_time(reqs, precision=64, limit_s=fp64_limit_s)
+    n = len(one["latency"])
+    steps64 = sum(r.traj_steps for r in one["results"])
+    out["fp64_one_at_a_time"] = {"requests_run": n, "service_s": float(one["service"].sum()),
+                                 "traj_steps_per_s": steps64 / max(float(one["service"].sum()), 1e-9),
+                                 "latency_s": S.percentiles(one["latency"]) if n else None,
+                                 "fp32_served_latency_s_same_requests": S.percentiles(res["latency"][:n]) if n else None}
+    return out
 
 
 def cpu_baseline(seconds=12.0):
     """The CPU restatement (oracle, lane mode, same Philox noise and step),
-    one process per host core, on a bounded sample of the same workload."""
+    one process per host core, on a bounded sample of the same workload: the
+    workers cycle over the 8 parameter sets, trajectories capped at 2e5 steps."""
     ncores = max(1, min(16, os.cpu_count() or 1))
     steps = 200000
     code = (
         "import sys, time, ctypes as C; sys.path.insert(0, %r); import oracle_lib as ol\n"
-        "lib = ol.load(); p = ol.make_params(model=0, overdamped=1, x0=-2.9, periodic_length=5.8, "
-        "heights=[[7.2, 2.0], [7.2, 2.0]], half_width=[0.65, 0.65], timestep=1e-4, tau=1e-5, nsteps=%d, "
+        "lib = ol.load(); w = int(sys.argv[1]); eb = (2.0, 3.0, 4.0, 5.0)[w %% 4]; over = (w // 4) %% 2 == 0\n"
+        "p = ol.make_params(model=0, overdamped=int(over), x0=-2.9, v0=0.0, randomize_v=0, periodic_length=5.8, "
+        "heights=[[3.6 * eb, eb], [3.6 * eb, eb]], half_width=[0.65, 0.65], timestep=1e-4, tau=1e-5, nsteps=%d, "
         "save_freq=%d, b_waiting=0, b_fpt=0, b_tpt=0, b_fptcg=0, absorb=1)\n"
-        "t0 = time.perf_counter(); n = 0; k = int(sys.argv[1])\n"
+        "t0 = time.perf_counter(); n = 0; k = w\n"
         "while time.perf_counter() - t0 < %f:\n"
-        "    out = ol.Lane(); lib.ldo_lane_run_philox(C.byref(p), C.byref(out), 1000, k, 0, 0, None, 0)\n"
+        "    out = ol.Lane(); lib.ldo_lane_run_philox(C.byref(p), C.byref(out), 1000 + w %% 4, k, 0, 0, None, 0)\n"
         "    n += out.steps_done; k += %d\n"
         "print(n, time.perf_counter() - t0)\n"
     ) % (os.path.join(REPO, "tests"), steps, steps, seconds, ncores)
@@ -94,25 +158,20 @@ def cpu_baseline(seconds=12.0):
         total += int(n)
         secs = max(secs, float(s))
     return {"value": total / secs, "unit": "traj-steps/s", "cores": ncores, "kind": "port",
-            "sample": f"oracle lane mode (scalar C, FP64, Philox noise), one process per core for ~{seconds:.0f} s "
-                      f"on the same workload parameters, trajectories absorbed or capped at {steps} steps"}
+            "sample": f"oracle lane mode (scalar C, FP64, Philox noise), one process per core for ~{seconds:.0f} s, "
+                      f"workers cycling over the 8 parameter sets, trajectories absorbed or capped at {steps} steps"}
 
 
 def pmc_child(args):
     """Counter-run body: a short run of the timed workload, then a plain copy of
     known size that calibrates the counters."""
-    sw, _ = time_engine(32, args.n_traj, 3, 1, args.steps_per_launch)
-    sw.close()
+    time_engine(32, args.n_traj, 3, 1, args.steps_per_launch, tune=False)
     import torch
-    n = COPY_ELEMS
-    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    a = torch.ones(COPY_ELEMS, device="cuda", dtype=torch.float32)
     b = torch.empty_like(a)
     for _ in range(3):
         torch.mul(a, 1.0, out=b)
     torch.cuda.synchronize()
-
-
-COPY_ELEMS = 1 << 26  # 256 MiB read + 256 MiB written per copy launch
 
 
 def read_counter_csv(out_dir, counter):
@@ -145,7 +204,7 @@ def pmc_traffic(args, timeout=240):
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="ldpcp_pmc_")
-        cmd = [prof, "--pmc", counter, "--kernel-include-regex", "sweep_step|elementwise", "-d", d, "-o", "pmc",
+        cmd = [prof, "--pmc", counter, "--kernel-include-regex", "engine_step|elementwise", "-d", d, "-o", "pmc",
                "-f", "csv", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
                "--n-traj", str(args.n_traj), "--steps-per-launch", str(args.steps_per_launch)]
         try:
@@ -154,12 +213,11 @@ def pmc_traffic(args, timeout=240):
             return None
         vals = read_counter_csv(d, counter)
         shutil.rmtree(d, ignore_errors=True)
-        step = [v for k, vs in vals.items() if "sweep_step" in k for v in vs]
+        step = [v for k, vs in vals.items() if "engine_step" in k for v in vs]
         copy = [v for k, vs in vals.items() if "elementwise" in k for v in vs]
         if len(step) < 3 or not copy:
             return None
-        known_kib = COPY_ELEMS * 4 / 1024.0
-        factor = known_kib / (sum(copy) / len(copy))
+        factor = (COPY_ELEMS * 4 / 1024.0) / (sum(copy) / len(copy))
         res[counter] = {"kib_per_launch": sum(step[-3:]) / 3, "copy_factor": factor,
                         "bytes_per_launch": sum(step[-3:]) / 3 * 1024.0 * factor}
     return res
@@ -168,16 +226,21 @@ def pmc_traffic(args, timeout=240):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--steps-per-launch", type=int, default=4096)
-    ap.add_argument("--n-traj", type=int, default=1 << 20)
-    ap.add_argument("--no-finish", action="store_true", help="skip the run to completion (MFPT check)")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps-per-launch", type=int, default=0, help="0 = the autotuner's choice")
+    ap.add_argument("--n-traj", type=int, default=1 << 20, help="trajectories per parameter set")
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--no-finish", action="store_true", help="skip the MFPT check")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
+        if not args.steps_per_launch:
+            args.steps_per_launch = STEPS_PER_LAUNCH
         pmc_child(args)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -194,27 +257,37 @@ def main():
     hbm = C.c_int64()
     _native.check(lib.ldpcp_device_info(name, 64, C.byref(ncu), C.byref(hbm)), "ldpcp_device_info")
 
-    spl = args.steps_per_launch
-    sw32, r32 = time_engine(32, args.n_traj, args.steps, args.warmup, spl)
+    r32 = time_engine(32, args.n_traj, args.steps, args.warmup, args.steps_per_launch)
     print(f"# fp32 {r32}", file=sys.stderr, flush=True)
-    sw64, r64 = time_engine(64, args.n_traj, args.steps, args.warmup, spl)
+    r64 = time_engine(64, args.n_traj, args.steps, args.warmup, args.steps_per_launch)
     print(f"# fp64 {r64}", file=sys.stderr, flush=True)
     v32 = r32["traj_steps"] / r32["wall_s"]
     v64 = r64["traj_steps"] / r64["wall_s"]
     k32 = r32["traj_steps"] / (r32["kernel_ms"] * 1e-3)
     k64 = r64["traj_steps"] / (r64["kernel_ms"] * 1e-3)
 
-    fin = {}
+    fin = None
     if not args.no_finish:
-        fin[32] = finish(sw32, spl)
-        print(f"# fp32 finish {fin[32]}", file=sys.stderr, flush=True)
-        fin[64] = finish(sw64, spl)
-        print(f"# fp64 finish {fin[64]}", file=sys.stderr, flush=True)
-    sw32.close()
-    sw64.close()
+        f32, t32 = mfpt_check(32, args.n_traj)
+        f64, t64 = mfpt_check(64, args.n_traj)
+        fin = {"sets": [], "seconds": {"f32": t32, "f64": t64}}
+        for a, b in zip(f32, f64):
+            fin["sets"].append({"overdamped": a["overdamped"], "mfpt_f32": a["mfpt"], "mfpt_f64": b["mfpt"],
+                                "rel_err": abs(a["mfpt"] - b["mfpt"]) / b["mfpt"], "stderr_f64": b["stderr"],
+                                "tpt_f32": a["tpt_mean"], "tpt_f64": b["tpt_mean"],
+                                "capped": [a["n_capped"], b["n_capped"]]})
+        fin["max_rel_err"] = max(s["rel_err"] for s in fin["sets"])
+        print(f"# mfpt {fin}", file=sys.stderr, flush=True)
+
+    serve = None
+    if not args.no_serve:
+        serve = serving(args.serve_seconds, args.serve_budget_gb)
+        print(f"# serving {serve}", file=sys.stderr, flush=True)
 
     traffic = None if args.no_traffic else pmc_traffic(args)
-    ops = OPS_PER_STEP["single_over"]
+    if not args.steps_per_launch:
+        args.steps_per_launch = r32["config"]["steps_per_launch"]
+    ops = r32["ops_per_step"]
     achieved = k32 * ops / 1e12
     peak = P_ISSUE / 1e12
     line = {
@@ -230,11 +303,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (counter-based Philox noise, no input files)",
-        "config": {"workload": "single-barrier overdamped first passage, Eb=2.0 (heights 7.2/2.0), "
-                               "L=5.8, w=0.65, dt=1e-4, tau=1e-5, 2^20 trajectories from X0=-2.9, "
-                               "cap 1e8 steps, Philox seed 1000",
-                   "n_traj": args.n_traj, "steps_per_launch": spl, "device": name.value.decode(),
-                   "compute_units": ncu.value},
+        "config": {"workload": "single-barrier first passage, 8 sets x 2^20 trajectories: Eb in {2,3,4,5} "
+                               "(heights 3.6Eb/Eb) x {overdamped, underdamped}, L=5.8, w=0.65, dt=1e-4, "
+                               "tau=1e-5, X0=-2.9 at rest, cap 1e8 steps, Philox seed 1000+Eb index",
+                   "n_traj_per_set": args.n_traj, "launch": r32["config"], "launch_fp64": r64["config"],
+                   "device": name.value.decode(), "compute_units": ncu.value},
         "fp64_value": v64,
         "fp32_over_fp64": v32 / v64,
         "kernel_traj_steps_per_s": {"f32": k32, "f64": k64},
@@ -243,18 +316,18 @@ def main():
                      "traffic": (traffic["FETCH_SIZE"]["bytes_per_launch"] + traffic["WRITE_SIZE"]["bytes_per_launch"])
                      if traffic else None,
                      "traffic_detail": traffic,
-                     "avg_launch_us": 1e3 * r32["kernel_ms"] / r32["launches"],
-                     "traj_steps_per_launch": r32["traj_steps"] / r32["launches"],
+                     "avg_launch_us": 1e3 * r32["kernel_ms"] / max(r32["launches"], 1),
+                     "traj_steps_per_launch": r32["traj_steps"] / max(r32["launches"], 1),
                      "ops_per_traj_step": ops,
-                     "note": "VALU issue bound (no MFMA, HBM traffic per step ~0); peak = 256 CU x 64 lanes x "
-                             "2.4 GHz non-packed issue; frac vs the 157.3 TFLOP/s packed-FP32 vector peak is "
-                             "achieved*2/157.3"},
+                     "note": "VALU issue bound (no MFMA; HBM traffic per trajectory-step ~0); peak = 256 CU x 64 "
+                             "lanes x 2.4 GHz non-packed issue; achieved = census ops per trajectory-step x "
+                             "trajectory-steps / launch time (HIP events around each launch: step, scan and "
+                             "scatter kernels); frac vs the 157.3 TFLOP/s packed-FP32 peak is achieved*2/157.3"},
     }
     if fin:
-        m32, m64 = fin[32]["mfpt"], fin[64]["mfpt"]
-        line["mfpt"] = {"f32": m32, "f64": m64, "rel_err": abs(m32 - m64) / m64,
-                        "stderr_f64": fin[64]["mfpt_stderr"], "tpt_f32": fin[32]["tpt_mean"],
-                        "tpt_f64": fin[64]["tpt_mean"], "capped": [fin[32]["n_capped"], fin[64]["n_capped"]]}
+        line["mfpt"] = fin
+    if serve:
+        line["serving"] = serve
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

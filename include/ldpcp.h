@@ -127,10 +127,12 @@ int ldpcp_run_msd(ldpcp_run* h, const uint64_t* lags, uint32_t nlags, double* ou
 int ldpcp_run_destroy(ldpcp_run* h);
 
 /* ------------------------------------------------------------------------- */
-/* Sweep engine: absorbing first-passage runs over ragged parameter sets.       */
+/* First-passage engine: absorbing runs over ragged parameter sets.             */
 /* No reference counterpart (the reference runs one parameter file per MPI     */
-/* job); it is the batched form of the same integrator, stopped at the first    */
-/* barrier-top crossing, with FPT/TPT/MSD statistics reduced on the device.     */
+/* job, langevinDynamicsSingleBarrier.h:3429-3589); it is the batched form of   */
+/* the same integrator, stopped at the first passage event, with FPT/TPT/MSD    */
+/* statistics reduced on the device.  Two front ends share it: a static sweep   */
+/* (all sets admitted at once) and a request server (continuous batching).      */
 /* ------------------------------------------------------------------------- */
 #define LDPCP_HIST_BINS 256
 #define LDPCP_MSD_LAGS 64
@@ -148,39 +150,101 @@ typedef struct ldpcp_sweep_set {
     int32_t equal_probability;
     double eq_constant;
     uint64_t n_traj;
-    uint64_t max_steps;            /* step cap per trajectory */
-    uint64_t seed;
+    uint64_t max_steps;            /* step cap per trajectory (< 2^32) */
+    uint64_t seed;                 /* Philox key */
 } ldpcp_sweep_set;
 
+/* Statistics of one parameter set.  The first passage event is the first
+ * barrier-top crossing for the single barrier (the reference's first FPT1
+ * sample) and the first arrival at a non-adjacent well centre for the double
+ * barrier (its first FPT2 sample).  Times are in the set's time units. */
 typedef struct ldpcp_set_stats {
     uint64_t n_traj, n_absorbed, n_capped;
     double fpt_sum, fpt_sum2;      /* first-passage times of absorbed trajectories */
-    double tpt_sum, tpt_sum2;      /* transition-path times of absorbed trajectories */
+    double tpt_sum, tpt_sum2;      /* transition-path times: last well-centre crossing -> event */
+    /* double barrier, per barrier: [0] last well-centre crossing -> first crossing
+     * of the intermediate-well bottom xc after it; [1] last crossing of xc -> event */
+    double tptb_sum[2], tptb_sum2[2];
     uint64_t traj_steps;           /* trajectory-steps integrated */
+    uint64_t n_finished;           /* absorbed + capped so far */
     double hist_lo, hist_hi;       /* log10 edges of the FPT histogram, in time units */
     uint64_t hist[LDPCP_HIST_BINS];
-    uint64_t msd_step[LDPCP_MSD_LAGS];
-    double msd_sum[LDPCP_MSD_LAGS];
+    uint64_t msd_step[LDPCP_MSD_LAGS]; /* lag steps (log-spaced in [1, max_steps]) */
+    double msd_sum[LDPCP_MSD_LAGS];    /* sum of (x(lag) - x0)^2 over trajectories alive at the lag */
     uint64_t msd_n[LDPCP_MSD_LAGS];
 } ldpcp_set_stats;
+
+/* Launch shape.  Zero fields take the defaults (block 256, 8 blocks per CU,
+ * 4096 steps per launch, graph replay on). */
+typedef struct ldpcp_engine_config {
+    int32_t block;                 /* threads per workgroup: 64, 128 or 256 */
+    int32_t blocks_per_cu;         /* persistent workgroups per compute unit */
+    uint32_t steps_per_launch;
+    int32_t use_graph;             /* 1: replay a captured hipGraph per launch, -1: plain launches */
+} ldpcp_engine_config;
 
 typedef struct ldpcp_sweep ldpcp_sweep;
 
 /* precision: 64 = plain FP64 stepping (the reference arithmetic), 32 = FP32
  * stepping with a compensated (cell, offset) position and an integer clock. */
 int ldpcp_sweep_create(const ldpcp_sweep_set* sets, int32_t nsets, int32_t precision, ldpcp_sweep** out);
-/* One launch over every tile that still has live trajectories; returns the
- * number of live trajectories left in *live. */
+int ldpcp_sweep_set_config(ldpcp_sweep* h, const ldpcp_engine_config* cfg);
+int ldpcp_sweep_get_config(ldpcp_sweep* h, ldpcp_engine_config* cfg);
+/* Autotuner: times candidate launch shapes on the sweep's own trajectories
+ * (results do not depend on the shape) and keeps the fastest. */
+int ldpcp_sweep_tune(ldpcp_sweep* h, ldpcp_engine_config* chosen);
+/* One launch of steps_per_launch steps (0 = the configured value) over every
+ * live trajectory; returns the number still live in *live. */
 int ldpcp_sweep_launch(ldpcp_sweep* h, uint32_t steps_per_launch, uint64_t* live);
 /* Launches until every trajectory is absorbed or capped. */
 int ldpcp_sweep_run(ldpcp_sweep* h, uint32_t steps_per_launch);
 int ldpcp_sweep_stats(ldpcp_sweep* h, int32_t set, ldpcp_set_stats* out);
-/* Per-trajectory absorption step (0 = capped) and last well-centre crossing. */
-int ldpcp_sweep_fpt(ldpcp_sweep* h, int32_t set, uint64_t* absorbed_step, uint64_t* tpt_start);
-/* Mean duration (ms) of the stepping kernel over the launches so far, timed
- * with HIP events on the handle's stream, and the launch count. */
+/* Per trajectory of a set: absorption step (0 = not absorbed), last
+ * well-centre crossing before it, and (double barrier) the first and last
+ * crossings of xc used by tptb.  Any output but absorbed_step may be NULL. */
+int ldpcp_sweep_fpt(ldpcp_sweep* h, int32_t set, uint64_t* absorbed_step, uint64_t* tpt_start, uint64_t* xc_first,
+                    uint64_t* xc_last);
+/* Total duration (ms) of the launches so far, timed with HIP events on the
+ * handle's stream around each launch, and the launch count. */
 int ldpcp_sweep_kernel_time(ldpcp_sweep* h, double* ms_total, uint64_t* launches);
 int ldpcp_sweep_destroy(ldpcp_sweep* h);
+
+/* ------------------------------------------------------------------------- */
+/* Request server: continuous batching of sweep requests on one GPU.           */
+/* Requests wait in FIFO order until their HBM footprint fits the budget, then */
+/* join the resident trajectories between two launches; absorbed trajectories  */
+/* are retired after every launch and a request completes when its last        */
+/* trajectory finishes.  A request's results do not depend on what else is    */
+/* resident.  One server per host thread.                                      */
+/* ------------------------------------------------------------------------- */
+typedef struct ldpcp_server ldpcp_server;
+
+typedef struct ldpcp_server_config {
+    uint64_t budget_bytes;         /* HBM for resident trajectory state + per-request statistics */
+    int32_t precision;             /* 32 or 64 */
+    int32_t max_requests;          /* resident requests at once (0 = 4096) */
+    ldpcp_engine_config engine;
+} ldpcp_server_config;
+
+enum { LDPCP_REQ_QUEUED = 0, LDPCP_REQ_RUNNING = 1, LDPCP_REQ_DONE = 2 };
+
+int ldpcp_server_create(const ldpcp_server_config* cfg, ldpcp_server** out);
+/* HBM bytes a request occupies while resident. */
+int ldpcp_server_footprint(const ldpcp_server* s, const ldpcp_sweep_set* set, uint64_t* bytes);
+int ldpcp_server_submit(ldpcp_server* s, const ldpcp_sweep_set* set, uint64_t* request_id);
+/* Admits what fits, runs up to n_launches launches (fewer if nothing is
+ * live), retires finished requests.  Returns without waiting for the last
+ * launch when wait == 0. */
+int ldpcp_server_pump(ldpcp_server* s, uint32_t n_launches, int32_t wait);
+int ldpcp_server_status(ldpcp_server* s, uint64_t request_id, int32_t* state);
+/* Ids of requests completed since the last call (at most max). */
+int ldpcp_server_completed(ldpcp_server* s, uint64_t* ids, int32_t max, int32_t* n);
+/* Results of a completed request; frees its record. */
+int ldpcp_server_result(ldpcp_server* s, uint64_t request_id, ldpcp_set_stats* out);
+/* queued / resident requests, live trajectories, resident bytes, trajectory-steps done */
+int ldpcp_server_info(ldpcp_server* s, uint64_t* queued, uint64_t* resident, uint64_t* live_traj,
+                      uint64_t* resident_bytes, uint64_t* traj_steps);
+int ldpcp_server_destroy(ldpcp_server* s);
 
 /* Library-wide. */
 const char* ldpcp_last_error(void);

@@ -88,7 +88,8 @@ class SetStats(C.Structure):
         ("n_traj", C.c_uint64), ("n_absorbed", C.c_uint64), ("n_capped", C.c_uint64),
         ("fpt_sum", C.c_double), ("fpt_sum2", C.c_double),
         ("tpt_sum", C.c_double), ("tpt_sum2", C.c_double),
-        ("traj_steps", C.c_uint64),
+        ("tptb_sum", C.c_double * 2), ("tptb_sum2", C.c_double * 2),
+        ("traj_steps", C.c_uint64), ("n_finished", C.c_uint64),
         ("hist_lo", C.c_double), ("hist_hi", C.c_double),
         ("hist", C.c_uint64 * HIST_BINS),
         ("msd_step", C.c_uint64 * MSD_LAGS),
@@ -96,6 +97,18 @@ class SetStats(C.Structure):
         ("msd_n", C.c_uint64 * MSD_LAGS),
     ]
 
+
+class EngineConfig(C.Structure):
+    _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("steps_per_launch", C.c_uint32),
+                ("use_graph", C.c_int32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("precision", C.c_int32), ("max_requests", C.c_int32),
+                ("engine", EngineConfig)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
 # (name, restype, argtypes) for every entry point of include/ldpcp.h
 _P = C.c_void_p
@@ -115,9 +128,23 @@ _SIGNATURES = [
     ("ldpcp_sweep_launch", C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("ldpcp_sweep_run", C.c_int, [_P, C.c_uint32]),
     ("ldpcp_sweep_stats", C.c_int, [_P, C.c_int32, C.POINTER(SetStats)]),
-    ("ldpcp_sweep_fpt", C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("ldpcp_sweep_set_config", C.c_int, [_P, C.POINTER(EngineConfig)]),
+    ("ldpcp_sweep_get_config", C.c_int, [_P, C.POINTER(EngineConfig)]),
+    ("ldpcp_sweep_tune", C.c_int, [_P, C.POINTER(EngineConfig)]),
+    ("ldpcp_sweep_fpt", C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                  C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("ldpcp_sweep_kernel_time", C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     ("ldpcp_sweep_destroy", C.c_int, [_P]),
+    ("ldpcp_server_create", C.c_int, [C.POINTER(ServerConfig), C.POINTER(_P)]),
+    ("ldpcp_server_footprint", C.c_int, [_P, C.POINTER(SweepSet), C.POINTER(C.c_uint64)]),
+    ("ldpcp_server_submit", C.c_int, [_P, C.POINTER(SweepSet), C.POINTER(C.c_uint64)]),
+    ("ldpcp_server_pump", C.c_int, [_P, C.c_uint32, C.c_int32]),
+    ("ldpcp_server_status", C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int32)]),
+    ("ldpcp_server_completed", C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int32, C.POINTER(C.c_int32)]),
+    ("ldpcp_server_result", C.c_int, [_P, C.c_uint64, C.POINTER(SetStats)]),
+    ("ldpcp_server_info", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("ldpcp_server_destroy", C.c_int, [_P]),
     ("ldpcp_last_error", C.c_char_p, []),
     ("ldpcp_device_info", C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
 ]

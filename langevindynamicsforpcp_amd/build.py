@@ -31,7 +31,7 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contr
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-I", os.path.join(REPO, "include")]
 
-LIB_SOURCES = ["ldpcp_common.cpp", "ldpcp_run.hip", "ldpcp_sweep.hip"]
+LIB_SOURCES = ["ldpcp_common.cpp", "ldpcp_run.hip", "ldpcp_engine.hip", "ldpcp_server.hip"]
 HOST_COMMON = ["input_parser.cpp", "runner.cpp", "text_util.cpp"]
 EXECUTABLES = {"langevinDynamicsSingleBarrier": ("main_single.cpp", 0),
                "langevinDynamicsDoubleBarrier": ("main_double.cpp", 1)}

@@ -1,4 +1,9 @@
-"""Host side of the batched first-passage engine (ldpcp_sweep_* in include/ldpcp.h)."""
+"""Host side of the batched first-passage engine (ldpcp_sweep_* in include/ldpcp.h).
+
+A sweep admits every parameter set at once and launches until every
+trajectory is absorbed or capped.  Results per trajectory depend only on its
+set, index and step count, never on the launch shape or on the other sets.
+"""
 import ctypes as C
 import math
 
@@ -47,12 +52,28 @@ class Sweep:
         self._h = h
         self.precision = precision
 
-    def launch(self, steps_per_launch):
+    def config(self):
+        c = N.EngineConfig()
+        N.check(N.load().ldpcp_sweep_get_config(self._h, C.byref(c)), "ldpcp_sweep_get_config")
+        return c
+
+    def set_config(self, block=0, blocks_per_cu=0, steps_per_launch=0, use_graph=0):
+        c = N.EngineConfig(block, blocks_per_cu, steps_per_launch, use_graph)
+        N.check(N.load().ldpcp_sweep_set_config(self._h, C.byref(c)), "ldpcp_sweep_set_config")
+
+    def tune(self):
+        """Autotune block size, persistent workgroups per CU and steps per launch
+        on this sweep's own trajectories; returns the chosen config."""
+        c = N.EngineConfig()
+        N.check(N.load().ldpcp_sweep_tune(self._h, C.byref(c)), "ldpcp_sweep_tune")
+        return c
+
+    def launch(self, steps_per_launch=0):
         live = C.c_uint64()
         N.check(N.load().ldpcp_sweep_launch(self._h, int(steps_per_launch), C.byref(live)), "ldpcp_sweep_launch")
         return live.value
 
-    def run(self, steps_per_launch=4096):
+    def run(self, steps_per_launch=0):
         N.check(N.load().ldpcp_sweep_run(self._h, int(steps_per_launch)), "ldpcp_sweep_run")
 
     def stats(self, i):
@@ -60,12 +81,18 @@ class Sweep:
         N.check(N.load().ldpcp_sweep_stats(self._h, int(i), C.byref(st)), "ldpcp_sweep_stats")
         return st
 
-    def fpt_steps(self, i):
+    def paths(self, i):
+        """Per trajectory: absorption step (0 = capped or live), last well-centre
+        crossing before it, first / last crossing of the intermediate-well bottom
+        (double barrier)."""
         n = self.sets[i].n_traj
-        a = np.zeros(n, np.uint64)
-        t = np.zeros(n, np.uint64)
-        N.check(N.load().ldpcp_sweep_fpt(self._h, int(i), a.ctypes.data_as(C.POINTER(C.c_uint64)),
-                                         t.ctypes.data_as(C.POINTER(C.c_uint64))), "ldpcp_sweep_fpt")
+        out = [np.zeros(n, np.uint64) for _ in range(4)]
+        ptr = [a.ctypes.data_as(C.POINTER(C.c_uint64)) for a in out]
+        N.check(N.load().ldpcp_sweep_fpt(self._h, int(i), *ptr), "ldpcp_sweep_fpt")
+        return out
+
+    def fpt_steps(self, i):
+        a, t, _, _ = self.paths(i)
         return a, t
 
     def kernel_time(self):

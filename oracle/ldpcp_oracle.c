@@ -1039,7 +1039,7 @@ static int lane_run(const ldo_params* p, ldo_lane* o, lane_noise* n, double* fra
 
     uint64_t F = p->save_freq;
     uint64_t nsteps = (F >= 10) ? (p->nsteps / F) * F : p->nsteps;
-    uint64_t last_period_step = 0;
+    uint64_t last_period_step = 0, xc_first = 0, xc_last = 0;
     /* absorbing mode 2 stops on entering a well centre outside the two cells
      * {a, a+1} the start lies between (the first FPT2 event, Single.h:2445-2489) */
     const int cell_a = (L.well == L.periodic) ? (int)L.periodic : (int)L.periodic - 1;
@@ -1086,13 +1086,25 @@ static int lane_run(const ldo_params* p, ldo_lane* o, lane_noise* n, double* fra
         if (!p->overdamped) L.v = L.v + (c->minv * L.F) * c->halfdt;
         stats_update(p, o, &L, iStep, NULL, NULL, 0, cap ? &cr : NULL);
         if (iStep % F == 0) lane_record(p, c, &L, frames, frames_cap, nf++, 1);
+        /* double barrier: crossings of the intermediate-well bottom xc (a local
+         * barrier change that is not a periodic-cell change) split the
+         * transition path into its barrier-1 and barrier-2 parts */
+        if (model == 1 && L.ent_local && !L.ent_period) {
+            xc_last = iStep;
+            if (!xc_first) xc_first = iStep;
+        }
         if ((p->absorb == 1 && L.ent_well) ||
             (p->absorb == 2 && L.ent_period && ((int)L.periodic - cell_a < 0 || (int)L.periodic - cell_a > 1))) {
             o->absorbed_step = iStep;
             o->tpt_start_step = last_period_step;
+            o->xc_first_step = xc_first;
+            o->xc_last_step = xc_last;
             break;
         }
-        if (L.ent_period) last_period_step = iStep;
+        if (L.ent_period) {
+            last_period_step = iStep;
+            xc_first = 0;
+        }
         if (n->err) break;
     }
     o->steps_done = iStep > nsteps ? nsteps : iStep;

@@ -69,6 +69,9 @@ typedef struct ldo_lane {
     /* absorbing mode */
     uint64_t absorbed_step;    /* 0 = not absorbed */
     uint64_t tpt_start_step;   /* last well-centre crossing before absorption */
+    uint64_t xc_first_step;    /* double barrier: first crossing of the intermediate-well
+                                  bottom xc after the last well-centre crossing (0 = none) */
+    uint64_t xc_last_step;     /* double barrier: last crossing of xc before absorption */
     uint64_t steps_done;
     /* noise consumption (lane mode) */
     uint64_t n_gauss, n_flip, n_catch;

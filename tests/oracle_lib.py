@@ -56,7 +56,8 @@ class Lane(C.Structure):
         ("wt_tick_prev", C.c_double), ("wt_passed", C.c_int),
         ("wt_ticks", C.c_uint64), ("wt_last", C.c_double), ("wt_sum", C.c_double), ("wt_sum2", C.c_double),
         ("wt_max", C.c_double), ("wt_n", C.c_uint64),
-        ("absorbed_step", C.c_uint64), ("tpt_start_step", C.c_uint64), ("steps_done", C.c_uint64),
+        ("absorbed_step", C.c_uint64), ("tpt_start_step", C.c_uint64),
+        ("xc_first_step", C.c_uint64), ("xc_last_step", C.c_uint64), ("steps_done", C.c_uint64),
         ("n_gauss", C.c_uint64), ("n_flip", C.c_uint64), ("n_catch", C.c_uint64),
         ("error", C.c_int),
     ]

@@ -50,20 +50,31 @@ def test_fp64_sweep_matches_oracle(oracle, model, over):
     print(f"[gpu sweep fp64] model={model} over={over}", flush=True)
     sw = Sweep([s], precision=64)
     sw.run(steps_per_launch=5000)
-    ab, tp = sw.fpt_steps(0)
+    ab, tp, xf, xl = sw.paths(0)
     st = sw.stats(0)
     same = 0
     for t in range(s.n_traj):
         ref = oracle_absorb(oracle, s, t, absorb)
-        if ref.absorbed_step == ab[t] and (ab[t] == 0 or ref.tpt_start_step == tp[t]):
-            same += 1
+        ok = ref.absorbed_step == ab[t] and (ab[t] == 0 or ref.tpt_start_step == tp[t])
+        if model == 1 and ab[t]:
+            ok = ok and ref.xc_first_step == xf[t] and ref.xc_last_step == xl[t]
+        same += ok
     assert same >= s.n_traj - 1, f"{s.n_traj - same} trajectories differ"
     assert st.n_absorbed == int(np.count_nonzero(ab))
     assert st.n_absorbed + st.n_capped == s.n_traj
     assert sum(st.hist) == st.n_absorbed
     assert st.n_absorbed > s.n_traj // 2
-    fpt = ab[ab > 0].astype(float) * s.timestep
+    m = ab > 0
+    fpt = ab[m].astype(float) * s.timestep
     assert st.fpt_sum == pytest.approx(fpt.sum(), rel=1e-12)
+    assert st.tpt_sum == pytest.approx(((ab[m] - tp[m]).astype(float) * s.timestep).sum(), rel=1e-12)
+    if model == 1:
+        # per-barrier transition paths: well centre -> xc, last xc crossing -> target
+        assert np.all(xf[m] > tp[m]) and np.all(xl[m] >= xf[m]) and np.all(xl[m] < ab[m])
+        b0 = ((xf[m] - tp[m]).astype(float) * s.timestep).sum()
+        b1 = ((ab[m] - xl[m]).astype(float) * s.timestep).sum()
+        assert st.tptb_sum[0] == pytest.approx(b0, rel=1e-12)
+        assert st.tptb_sum[1] == pytest.approx(b1, rel=1e-12)
     sw.close()
 
 
