@@ -77,7 +77,8 @@ def time_engine(precision, n_traj, steps, warmup, spl, tune=True):
     d = [b - a for a, b in zip(per_set0, per_set1)]
     ops = sum(di * OPS_PER_STEP[(s.model, bool(s.overdamped))] for di, s in zip(d, sets)) / max(sum(d), 1)
     c = sw.config()
-    res = {"traj_steps": sum(d), "wall_s": wall, "kernel_ms": ms1 - ms0, "launches": n1 - n0, "ops_per_step": ops,
+    res = {"traj_steps": sum(d), "total_traj_steps": sum(per_set1), "total_launches": n1,
+           "wall_s": wall, "kernel_ms": ms1 - ms0, "launches": n1 - n0, "ops_per_step": ops,
            "config": {"block": c.block, "blocks_per_cu": c.blocks_per_cu, "steps_per_launch": c.steps_per_launch,
                       "use_graph": c.use_graph, "tuned": cfg is not None}}
     sw.close()
@@ -208,14 +209,18 @@ def pmc_traffic(args, timeout=240):
                "-f", "csv", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
                "--n-traj", str(args.n_traj), "--steps-per-launch", str(args.steps_per_launch)]
         try:
-            subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=timeout, check=False)
+            pr = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=timeout,
+                                check=False)
         except subprocess.TimeoutExpired:
+            print(f"# traffic: {counter} run timed out", file=sys.stderr, flush=True)
             return None
         vals = read_counter_csv(d, counter)
         shutil.rmtree(d, ignore_errors=True)
         step = [v for k, vs in vals.items() if "engine_step" in k for v in vs]
         copy = [v for k, vs in vals.items() if "elementwise" in k for v in vs]
         if len(step) < 3 or not copy:
+            print(f"# traffic: {counter} run gave {len(step)} step and {len(copy)} copy dispatches (rc {pr.returncode}); "
+                  f"kernels {sorted(vals)[:6]}; output tail:\n{pr.stdout[-1500:]}", file=sys.stderr, flush=True)
             return None
         factor = (COPY_ELEMS * 4 / 1024.0) / (sum(copy) / len(copy))
         res[counter] = {"kib_per_launch": sum(step[-3:]) / 3, "copy_factor": factor,
@@ -233,6 +238,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-finish", action="store_true", help="skip the MFPT check")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of the autotuner's")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs")
@@ -257,9 +263,9 @@ def main():
     hbm = C.c_int64()
     _native.check(lib.ldpcp_device_info(name, 64, C.byref(ncu), C.byref(hbm)), "ldpcp_device_info")
 
-    r32 = time_engine(32, args.n_traj, args.steps, args.warmup, args.steps_per_launch)
+    r32 = time_engine(32, args.n_traj, args.steps, args.warmup, args.steps_per_launch, tune=not args.no_tune)
     print(f"# fp32 {r32}", file=sys.stderr, flush=True)
-    r64 = time_engine(64, args.n_traj, args.steps, args.warmup, args.steps_per_launch)
+    r64 = time_engine(64, args.n_traj, args.steps, args.warmup, args.steps_per_launch, tune=not args.no_tune)
     print(f"# fp64 {r64}", file=sys.stderr, flush=True)
     v32 = r32["traj_steps"] / r32["wall_s"]
     v64 = r64["traj_steps"] / r64["wall_s"]
