@@ -62,6 +62,8 @@ struct SetDev {
     float fxb[2], flo[2], fhi[2], fxc, fslope[2][2][2];
     float fxp0, fpad;
     uint64_t T_flip, T_asym[2], T_create, T_p0le;  // integer thresholds on Philox words
+    uint32_t Tl[2];                                 // per-step flip thresholds leaving state 0 / 1, below 2^32
+    int32_t Tall[2];                                // 1: that threshold is 2^32 (always flip)
     int32_t fixed_state, cell0;                     // initial state when not randomised; cell of X0
     uint32_t key0, key1;
     int32_t slot_base, well0;                       // first of the two cells (slot 0)
@@ -128,7 +130,8 @@ __device__ __forceinline__ double uni(double v)
 struct Hot {
     int32_t target, equal_prob, tau_zero, slot_base;
     uint32_t key0, key1, first, max_steps;
-    uint64_t T_flip, T_asym0, T_asym1;
+    uint32_t Tl0, Tl1;    // per-step flip thresholds for leaving state 0 / 1 ...
+    int32_t all0, all1;   // ... or 1 when the threshold is 2^32 (always flip)
     // FP32 engine
     float fL, fhalfL, fdtgm, fminv_halfdt, fhalfdt, fexpmgdt, fvar, fw, fS0, fS1, fxc;
     float fxb[2], flo[2], fhi[2], fsl[2][2][2];  // fsl[barrier][state][side]
@@ -149,9 +152,10 @@ __device__ __forceinline__ Hot load_hot(const SetDev* __restrict__ g)
     h.key1 = uni(g->key1);
     h.first = uni((uint32_t)g->first);
     h.max_steps = uni((uint32_t)g->max_steps);
-    h.T_flip = uni(g->T_flip);
-    h.T_asym0 = uni(g->T_asym[0]);
-    h.T_asym1 = uni(g->T_asym[1]);
+    h.Tl0 = uni(g->Tl[0]);
+    h.Tl1 = uni(g->Tl[1]);
+    h.all0 = uni(g->Tall[0]);
+    h.all1 = uni(g->Tall[1]);
     if (F32) {
         h.fL = uni(g->fL);
         h.fhalfL = uni(g->fhalfL);
@@ -296,14 +300,20 @@ __device__ __forceinline__ int handover(const Hot& H, const SetDev* __restrict__
     return flip ? (prev ^ 1) : prev;
 }
 
-// per-step flip (langevinDynamicsSingleBarrier.h:2010-2015): thresholds T0 / T1
-// for leaving state 0 / 1 (equal when the two states are equally probable),
-// passed as values so that no threshold is selected through memory
-__device__ __forceinline__ int flip_step(bool tau_zero, uint64_t T0, uint64_t T1, int state, uint32_t word)
+// per-step flip (langevinDynamicsSingleBarrier.h:2010-2015), u < p as the
+// integer test w < T.  EQ: equally probable states share one threshold.
+template <bool EQ>
+__device__ __forceinline__ int flip_step(const Hot& H, int state, uint32_t word)
 {
-    if (tau_zero) return state;
-    const uint64_t T = state ? T1 : T0;
-    return below(word, T) ? (state ^ 1) : state;
+    if (H.tau_zero) return state;
+    bool f;
+    if (EQ) {
+        f = H.all0 || word < H.Tl0;
+    } else {
+        const uint32_t T = state ? H.Tl1 : H.Tl0;
+        f = (state ? H.all1 : H.all0) || word < T;
+    }
+    return f ? (state ^ 1) : state;
 }
 
 // Launch window of one set: local steps s0+1 .. s_end, MSD lags [j0, j1).
@@ -319,11 +329,12 @@ template <int MODEL>
 __device__ __forceinline__ float force_f32(const Hot& H, float xp, int state)
 {
     if (MODEL == 0) {
+        // -slope on (-w, 0), +slope on (0, w), 0 elsewhere (strict compares,
+        // langevinDynamicsSingleBarrier.h:1292-1324): the slope with xp's sign bit
+        // xor-ed in, where 0 < |xp| < w
         const float sp = state ? H.fS1 : H.fS0;
-        float F = 0.0f;
-        F = (xp < 0.0f && xp > -H.fw) ? -sp : F;
-        F = (xp < H.fw && xp > 0.0f) ? sp : F;
-        return F;
+        const float signed_sp = __uint_as_float(__float_as_uint(sp) ^ (__float_as_uint(xp) & 0x80000000u));
+        return (fabsf(xp) < H.fw && xp != 0.0f) ? signed_sp : 0.0f;
     } else {
         const float a0 = state ? H.fsl[0][1][0] : H.fsl[0][0][0], a1 = state ? H.fsl[0][1][1] : H.fsl[0][0][1];
         const float b0 = state ? H.fsl[1][1][0] : H.fsl[1][0][0], b1 = state ? H.fsl[1][1][1] : H.fsl[1][0][1];
@@ -336,8 +347,9 @@ __device__ __forceinline__ float force_f32(const Hot& H, float xp, int state)
     }
 }
 
-// One trajectory for one launch window.  S is a wave-uniform copy of the set.
-template <int MODEL, bool OVER>
+// One trajectory for one launch window.  H holds the set's hot constants in
+// scalar registers; EQ: the two barrier states are equally probable.
+template <int MODEL, bool OVER, bool EQ>
 __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St, const Hot H,
                                          const SetDev* __restrict__ Sg, int set, uint32_t slot, const Window W,
                                          double* lds_sum, uint32_t* lds_n, bool& fin, uint32_t& nsteps)
@@ -357,16 +369,17 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
     }
     int state = flags & F_STATE;
     int local = (MODEL == 1) ? (xp < H.fxc ? 0 : 1) : 0;
+    bool neg = xp < 0.0f;
     float F = force_f32<MODEL>(H, xp, state);
     int j = W.j0;
     uint32_t next_lag = j < LDPCP_MSD_LAGS ? Sg->lag[j] : kNoLag;
-    const uint64_t T0 = H.equal_prob ? H.T_flip : H.T_asym0;
-    const uint64_t T1 = H.equal_prob ? H.T_flip : H.T_asym1;
     float g_odd = 0.0f;
     uint32_t w_odd = 0;
+    // the step counter is wave-uniform; the exit step of a lane is kept apart
+    // so that the counter is not live after the loop (it stays scalar)
+    uint32_t last = W.s_end;
     bool done = false;
-    uint32_t step;
-    for (step = W.s0 + 1; step <= W.s_end; ++step) {
+    for (uint32_t step = W.s0 + 1; step <= W.s_end; ++step) {
         float g;
         uint32_t wflip;
         if ((step & 1u) == 1u) {
@@ -402,22 +415,21 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
         const float y = dx - comp;
         const float t = xp + y;
         comp = (t - xp) - y;
-        const bool neg_before = xp < 0.0f;
         xp = t;
-        // periodic (well-centre) and well (barrier-top) crossings
+        // periodic (well-centre) crossing: offset kept in [-L/2, L/2)
         bool ent_period = false;
         const int cell_prev = cell;
-        if (xp >= H.fhalfL) {
-            xp -= H.fL;
-            cell += 1;
-            ent_period = true;
-        } else if (xp < -H.fhalfL) {
-            xp += H.fL;
-            cell -= 1;
+        if (__builtin_expect(xp >= H.fhalfL || xp < -H.fhalfL, 0)) {
+            const bool up = xp >= H.fhalfL;
+            xp = up ? xp - H.fL : xp + H.fL;
+            cell = up ? cell + 1 : cell - 1;
             ent_period = true;
         }
-        const bool ent_well = !ent_period && (neg_before != (xp < 0.0f));
-        state = flip_step(H.tau_zero, T0, T1, state, wflip);
+        // well (barrier-top) crossing: the sign of the offset changes within a cell
+        const bool neg_before = neg;
+        neg = xp < 0.0f;
+        const bool ent_well = !ent_period && (neg_before != neg);
+        state = flip_step<EQ>(H, state, wflip);
         if (MODEL == 0) {
             if (ent_period) {
                 const int kp = slot_index(0, cell_prev, H.slot_base, 0);
@@ -447,11 +459,14 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
             ++j;
             next_lag = j < LDPCP_MSD_LAGS ? Sg->lag[j] : kNoLag;
         }
+        // single barrier: first barrier-top crossing; double: first well centre
+        // outside the two cells of the start
         const bool hit =
-            (H.target == 0) ? ent_well : (ent_period && (cell - H.slot_base < 0 || cell - H.slot_base > 1));
+            (MODEL == 0) ? ent_well : (ent_period && (unsigned)(cell - H.slot_base) > 1u);
         if (hit) {
             record_absorption<MODEL>(Sg, P, St, set, slot, step, tpt0, xf, xl);
             done = true;
+            last = step;
             break;
         }
         if (ent_period) {
@@ -459,7 +474,6 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
             if (MODEL == 1) xf = 0;
         }
     }
-    const uint32_t last = done ? step : W.s_end;
     if (!done && W.s_end >= H.max_steps) done = true;  // capped
     if (done) flags &= ~F_ALIVE;
     flags = (flags & ~F_STATE) | (uint32_t)state;
@@ -504,7 +518,7 @@ __device__ __forceinline__ double force_f64(const Hot& H, double x, double perio
     }
 }
 
-template <int MODEL, bool OVER>
+template <int MODEL, bool OVER, bool EQ>
 __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St, const Hot H,
                                          const SetDev* __restrict__ Sg, int set, uint32_t slot, const Window W,
                                          double* lds_sum, uint32_t* lds_n, bool& fin, uint32_t& nsteps)
@@ -528,13 +542,11 @@ __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St,
     double F = force_f64<MODEL>(H, x, periodic, state);
     int j = W.j0;
     uint32_t next_lag = j < LDPCP_MSD_LAGS ? Sg->lag[j] : kNoLag;
-    const uint64_t T0 = H.equal_prob ? H.T_flip : H.T_asym0;
-    const uint64_t T1 = H.equal_prob ? H.T_flip : H.T_asym1;
     double g_odd = 0.0;
     uint32_t w_odd = 0;
+    uint32_t last = W.s_end;
     bool done = false;
-    uint32_t step;
-    for (step = W.s0 + 1; step <= W.s_end; ++step) {
+    for (uint32_t step = W.s0 + 1; step <= W.s_end; ++step) {
         double g;
         uint32_t wflip;
         if ((step & 1u) == 1u) {
@@ -572,7 +584,7 @@ __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St,
         periodic = rint(r);
         const bool ent_well = well != well_prev;
         const bool ent_period = periodic != periodic_prev;
-        state = flip_step(H.tau_zero, T0, T1, state, wflip);
+        state = flip_step<EQ>(H, state, wflip);
         if (MODEL == 0) {
             if (ent_period) {
                 const int kp = slot_index(0, (int)periodic_prev, H.slot_base, 0);
@@ -602,10 +614,11 @@ __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St,
             next_lag = j < LDPCP_MSD_LAGS ? Sg->lag[j] : kNoLag;
         }
         const int kcell = (int)periodic - H.slot_base;
-        const bool hit = (H.target == 0) ? ent_well : (ent_period && (kcell < 0 || kcell > 1));
+        const bool hit = (MODEL == 0) ? ent_well : (ent_period && (kcell < 0 || kcell > 1));
         if (hit) {
             record_absorption<MODEL>(Sg, P, St, set, slot, step, tpt0, xf, xl);
             done = true;
+            last = step;
             break;
         }
         if (ent_period) {
@@ -613,7 +626,6 @@ __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St,
             if (MODEL == 1) xf = 0;
         }
     }
-    const uint32_t last = done ? step : W.s_end;
     if (!done && W.s_end >= H.max_steps) done = true;
     if (done) flags &= ~F_ALIVE;
     flags = (flags & ~F_STATE) | (uint32_t)state;
@@ -637,12 +649,16 @@ __device__ __forceinline__ void body(const SlotPtrs& P, const StatsPtrs& St, con
                                      uint32_t slot, const Window W, double* lds_sum, uint32_t* lds_n, bool& fin,
                                      uint32_t& nsteps)
 {
-    if (F32)
-        body_f32<MODEL, OVER>(P, St, load_hot<true, MODEL, OVER>(Sg), Sg, set, slot, W, lds_sum, lds_n, fin,
-                              nsteps);
-    else
-        body_f64<MODEL, OVER>(P, St, load_hot<false, MODEL, OVER>(Sg), Sg, set, slot, W, lds_sum, lds_n, fin,
-                              nsteps);
+    const bool eq = __builtin_amdgcn_readfirstlane(Sg->equal_prob) != 0;
+    if (F32) {
+        const Hot H = load_hot<true, MODEL, OVER>(Sg);
+        if (eq) body_f32<MODEL, OVER, true>(P, St, H, Sg, set, slot, W, lds_sum, lds_n, fin, nsteps);
+        else body_f32<MODEL, OVER, false>(P, St, H, Sg, set, slot, W, lds_sum, lds_n, fin, nsteps);
+    } else {
+        const Hot H = load_hot<false, MODEL, OVER>(Sg);
+        if (eq) body_f64<MODEL, OVER, true>(P, St, H, Sg, set, slot, W, lds_sum, lds_n, fin, nsteps);
+        else body_f64<MODEL, OVER, false>(P, St, H, Sg, set, slot, W, lds_sum, lds_n, fin, nsteps);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1037,6 +1053,11 @@ static void fill_set_dev(const ldpcp_sweep_set& s, uint64_t first, uint64_t star
     d.T_asym[0] = flip_threshold(d.p_asym[0]);
     d.T_asym[1] = flip_threshold(d.p_asym[1]);
     d.T_create = flip_threshold(d.p_create);
+    for (int k = 0; k < 2; ++k) {
+        const uint64_t T = s.equal_probability ? d.T_flip : d.T_asym[k];
+        d.Tall[k] = T >= (1ull << 32) ? 1 : 0;
+        d.Tl[k] = d.Tall[k] ? 0u : (uint32_t)T;
+    }
     if (s.randomize_state) {
         // u <= p0  <=>  w <= floor(p0 2^32 - 1/2)
         const double P = floor(d.p0 * 4294967296.0 - 0.5) + 1.0;
