@@ -237,7 +237,9 @@ def pmc_traffic(args, timeout=240):
         vals = read_counter_csv(d, counter)
         shutil.rmtree(d, ignore_errors=True)
         step = [v for k, vs in vals.items() if "engine_step" in k for v in vs]
-        copy = [v for k, vs in vals.items() if "copyBuffer" in k for v in vs]
+        # the runtime's copy kernel also serves the engine's small host copies:
+        # the calibration copies are the last three dispatches
+        copy = [v for k, vs in vals.items() if "copyBuffer" in k for v in vs][-3:]
         if len(step) < 3 or not copy:
             print(f"# traffic: {counter} run gave {len(step)} step and {len(copy)} copy dispatches (rc {pr.returncode}); "
                   f"kernels {sorted(vals)[:6]}; output tail:\n{pr.stdout[-1500:]}", file=sys.stderr, flush=True)
@@ -245,6 +247,12 @@ def pmc_traffic(args, timeout=240):
         factor = (COPY_BYTES / 1024.0) / (sum(copy) / len(copy))
         res[counter] = {"kib_per_launch": sum(step[-3:]) / 3, "copy_factor": factor,
                         "bytes_per_launch": sum(step[-3:]) / 3 * 1024.0 * factor}
+    # what the launch must move: every live trajectory's state read once and
+    # written once (FP32 single barrier: 56 B read -- live-list entry, set,
+    # 5 state words, flags, 4 barrier steps, catch-up count, TPT start -- and
+    # 48 B written)
+    n_live = len(workload_sets(1)) * args.n_traj
+    res["algorithmic_bytes_per_launch"] = {"read": 56 * n_live, "write": 48 * n_live}
     return res
 
 
