@@ -3,6 +3,7 @@
 // langevinDynamicsDoubleBarrier.cpp:80-85 and :148-157).
 #pragma once
 #include <stdio.h>
+#include <stdlib.h>
 
 #include "input_parser.h"
 #include "runner.h"
@@ -134,8 +135,21 @@ inline RunnerConfig read_config(const InputParser& ip, int model)
     return c;
 }
 
+// Under an MPI launcher (mpirun -n R ...), rank 0 runs all R emulated ranks on
+// the GPU (their seeds, layout and reductions are the reference's); the other
+// launched processes have nothing to do and exit quietly.
+inline int launcher_rank()
+{
+    for (const char* name : {"OMPI_COMM_WORLD_RANK", "PMI_RANK", "PMIX_RANK", "LDPCP_RANK"}) {
+        const char* v = getenv(name);
+        if (v && *v) return atoi(v);
+    }
+    return 0;
+}
+
 inline int program_main(int argc, char** argv, int model)
 {
+    if (launcher_rank() > 0) return 0;
     InputParser ip;
     add_options(ip, model);
     if (!ip.Parse(argc, argv, true)) return 0;
