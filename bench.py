@@ -12,7 +12,12 @@ launch.  A bench "step" is one launch (steps_per_launch integration steps of
 every live trajectory).  The FP32+compensated engine is timed (value) next to
 the plain FP64 engine, both with the autotuned launch shape.  Afterwards the
 Eb = 2 sets run to completion in both arithmetics and their mean first-passage
-times are compared.
+times are compared (2^20 overdamped, 2^18 underdamped trajectories).
+
+Double barrier: 8 sets (Eb in {2,3,4,5} x {over, under}, sample_input/CuIDB_double
+geometry) of 2^22 trajectories, FP32 throughput and per-barrier transition
+paths.  Ragged: 256 parameter sets drawn from seed 7 (sweep.ragged_sets) run to
+completion in FP32 and FP64, per-set statistics compared.
 
 Serving: a Poisson stream of sweep requests (20 per second for --serve-seconds,
 seed 11; single or double barrier, under- or overdamped, Eb ~ U[2, 6.5] with
@@ -92,6 +97,9 @@ def mfpt_check(precision, n_traj):
     """Eb = 2 sets (overdamped, underdamped) run to completion."""
     from langevindynamicsforpcp_amd.sweep import Sweep
     sets = workload_sets(n_traj, ebs=(2.0,))
+    # the underdamped set's MFPT is 15x the overdamped one's: a quarter of the
+    # trajectories keeps the run to completion short
+    sets[1].n_traj = max(n_traj // 4, 1)
     sw = Sweep(sets, precision=precision)
     sw.tune()
     t0 = time.perf_counter()
@@ -104,6 +112,81 @@ def mfpt_check(precision, n_traj):
                     "n_capped": st.n_capped, "tpt_mean": st.tpt_sum / max(st.n_absorbed, 1)})
     sw.close()
     return out, time.perf_counter() - t0
+
+
+def double_barrier(n_traj, steps, warmup):
+    """Double-barrier sets (sample_input/CuIDB_double geometry: barrier 0
+    between 1.7 Eb and Eb, barrier 1 between 3.6 Eb and Eb), Eb in {2,3,4,5}
+    x {overdamped, underdamped}, n_traj each, FP32: throughput over `steps`
+    launches after the autotuner and `warmup` launches, and the per-barrier
+    transition paths of the trajectories absorbed by then."""
+    from langevindynamicsforpcp_amd.sweep import Sweep, double_barrier_set
+    sets = [double_barrier_set(eb, overdamped=over, n_traj=n_traj, seed=1000 + k)
+            for over in (True, False) for k, eb in enumerate(EBS)]
+    sw = Sweep(sets, precision=32)
+    c = sw.tune()
+    for _ in range(warmup):
+        sw.launch()
+    s0 = [sw.stats(i).traj_steps for i in range(len(sets))]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sw.launch()
+    wall = time.perf_counter() - t0
+    st = [sw.stats(i) for i in range(len(sets))]
+    sw.close()
+    d = [b.traj_steps - a for a, b in zip(s0, st)]
+    ops = sum(di * OPS_PER_STEP[(1, bool(x.overdamped))] for di, x in zip(d, sets)) / max(sum(d), 1)
+    per_set = []
+    for x, y in zip(sets, st):
+        n = max(y.n_absorbed, 1)
+        per_set.append({"eb": x.height[0][1], "overdamped": bool(x.overdamped), "absorbed": y.n_absorbed,
+                        "mean_fpt_absorbed": y.fpt_sum / n, "mean_tpt": y.tpt_sum / n,
+                        "mean_tpt_barrier0": y.tptb_sum[0] / n, "mean_tpt_barrier1": y.tptb_sum[1] / n})
+    return {"n_traj_per_set": n_traj, "sets": len(sets), "traj_steps_per_s": sum(d) / wall, "wall_s": wall,
+            "ops_per_traj_step": ops, "launch": {"block": c.block, "blocks_per_cu": c.blocks_per_cu,
+                                                  "steps_per_launch": c.steps_per_launch},
+            "per_set": per_set}
+
+
+def ragged(max_steps=200000, seed=7):
+    """256 ragged parameter sets (sweep.ragged_sets, seed 7) run to completion
+    (absorbed or capped at max_steps) in FP32 and in FP64, each with the
+    autotuned launch shape; per set, the absorbed fraction and the MFPT of the
+    absorbed trajectories are compared between the two."""
+    import numpy as np
+    from langevindynamicsforpcp_amd.sweep import Sweep, ragged_sets
+    sets = ragged_sets(seed=seed, max_steps=max_steps)
+    out, st = {}, {}
+    for prec in (32, 64):
+        sw = Sweep(sets, precision=prec)
+        c = sw.tune()
+        t0 = time.perf_counter()
+        sw.run()
+        wall = time.perf_counter() - t0
+        st[prec] = [sw.stats(i) for i in range(len(sets))]
+        sw.close()
+        steps = sum(x.traj_steps for x in st[prec])
+        out[f"f{prec}"] = {"traj_steps": steps, "wall_s_after_tune": wall,
+                           "launch": {"block": c.block, "blocks_per_cu": c.blocks_per_cu,
+                                      "steps_per_launch": c.steps_per_launch}}
+    z_frac, rel_mfpt, z_mfpt = [], [], []
+    for x, a, b in zip(sets, st[32], st[64]):
+        n = x.n_traj
+        pa, pb = a.n_absorbed / n, b.n_absorbed / n
+        se = math.sqrt(max(pb * (1 - pb), 1.0 / n) / n)
+        z_frac.append(abs(pa - pb) / se)
+        if a.n_absorbed >= 1000 and b.n_absorbed >= 1000:
+            ma, mb = a.fpt_sum / a.n_absorbed, b.fpt_sum / b.n_absorbed
+            var = max(b.fpt_sum2 / b.n_absorbed - mb * mb, 0.0)
+            rel_mfpt.append(abs(ma - mb) / mb)
+            z_mfpt.append(abs(ma - mb) / max(math.sqrt(var / b.n_absorbed), 1e-300))
+    return {"sets": len(sets), "trajectories": sum(x.n_traj for x in sets), "max_steps": max_steps, "seed": seed,
+            **out, "fp32_over_fp64_wall": out["f64"]["wall_s_after_tune"] / out["f32"]["wall_s_after_tune"],
+            "check_vs_fp64": {"max_z_absorbed_fraction": float(max(z_frac)),
+                              "sets_with_mfpt": len(rel_mfpt),
+                              "max_rel_err_mfpt": float(max(rel_mfpt)) if rel_mfpt else None,
+                              "median_rel_err_mfpt": float(np.median(rel_mfpt)) if rel_mfpt else None,
+                              "max_z_mfpt": float(max(z_mfpt)) if z_mfpt else None}}
 
 
 def serving(seconds, budget_gb, rate=20.0, seed=11, fp64_limit_s=60.0):
@@ -268,6 +351,8 @@ def main():
     ap.add_argument("--no-finish", action="store_true", help="skip the MFPT check")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of the autotuner's")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-double", action="store_true", help="skip the double-barrier sets (2^22 trajectories each)")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the 256 ragged parameter sets")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -316,6 +401,15 @@ def main():
                                 "capped": [a["n_capped"], b["n_capped"]]})
         fin["max_rel_err"] = max(s["rel_err"] for s in fin["sets"])
         print(f"# mfpt {fin}", file=sys.stderr, flush=True)
+
+    dbl = None
+    if not args.no_double:
+        dbl = double_barrier(4 * args.n_traj, 3, 1)
+        print(f"# double barrier {dbl}", file=sys.stderr, flush=True)
+    rag = None
+    if not args.no_ragged:
+        rag = ragged()
+        print(f"# ragged {rag}", file=sys.stderr, flush=True)
 
     serve = None
     if not args.no_serve:
@@ -366,6 +460,10 @@ def main():
     }
     if fin:
         line["mfpt"] = fin
+    if dbl:
+        line["double_barrier"] = dbl
+    if rag:
+        line["ragged"] = rag
     if serve:
         line["serving"] = serve
     if not args.no_cpu_baseline:

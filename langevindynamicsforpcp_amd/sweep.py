@@ -40,6 +40,49 @@ def sweep_set(model=0, overdamped=True, eb=2.0, ratio=3.6, n_traj=1 << 20, max_s
     return s
 
 
+def double_barrier_set(eb, overdamped=True, n_traj=1 << 22, max_steps=10 ** 8, seed=1000, timestep=1e-4, tau=1e-5,
+                       ratio0=1.7, ratio1=3.6, **kw):
+    """sample_input/CuIDB_double geometry: L = 5.8, barriers at -1.42 / -0.56
+    with half widths 0.70 / 0.65, barrier 0 alternating between ratio0*Eb and
+    Eb, barrier 1 between ratio1*Eb and Eb, X0 = -2.9 (outer-well centre)."""
+    return sweep_set(model=1, overdamped=overdamped, heights=[[ratio0 * eb, eb], [ratio1 * eb, eb]],
+                     n_traj=n_traj, max_steps=max_steps, seed=seed, x0=-2.9, periodic_length=5.8,
+                     half_width=(0.70, 0.65), position=(-1.42, -0.56), timestep=timestep, tau=tau, **kw)
+
+
+def ragged_sets(seed=7, n_sets=256, max_steps=200000, n_lo=2 ** 10, n_hi=2 ** 18, timestep=1e-4):
+    """A ragged batch of parameter sets: Eb ~ U[2, 6.5]; amplitude ratio 2.7
+    (sample_input/CuOPTz geometry: L = 2.1, w = 0.6, X0 = -1.05) or 3.6
+    (sample_input/CuIDB_single: L = 5.8, w = 0.65, X0 = -2.9); tau
+    log-uniform in [1e-3, 1e2]; gamma log-uniform in [0.1, 10]; single or
+    double barrier (CuIDB_double geometry) with p = 0.5; under- or overdamped
+    with p = 0.5; trajectory counts log-uniform in [n_lo, n_hi]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_sets):
+        eb = float(rng.uniform(2.0, 6.5))
+        ratio = float(rng.choice([2.7, 3.6]))
+        tau = float(math.exp(rng.uniform(math.log(1e-3), math.log(1e2))))
+        gamma = float(math.exp(rng.uniform(math.log(0.1), math.log(10.0))))
+        model = int(rng.integers(2))
+        over = bool(rng.integers(2))
+        n = int(round(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi)))))
+        sd = int(rng.integers(1, 2 ** 62))
+        if model == 1:
+            s = double_barrier_set(eb, overdamped=over, n_traj=n, max_steps=max_steps, seed=sd, timestep=timestep,
+                                   tau=tau, ratio1=ratio, gamma=gamma, randomize_v=not over)
+        elif ratio == 2.7:
+            s = sweep_set(model=0, overdamped=over, eb=eb, ratio=ratio, n_traj=n, max_steps=max_steps, seed=sd,
+                          x0=-1.05, periodic_length=2.1, half_width=(0.6, 0.6), timestep=timestep, tau=tau,
+                          gamma=gamma, randomize_v=not over)
+        else:
+            s = sweep_set(model=0, overdamped=over, eb=eb, ratio=ratio, n_traj=n, max_steps=max_steps, seed=sd,
+                          x0=-2.9, periodic_length=5.8, half_width=(0.65, 0.65), timestep=timestep, tau=tau,
+                          gamma=gamma, randomize_v=not over)
+        out.append(s)
+    return out
+
+
 class Sweep:
     """Absorbing first-passage runs over a list of parameter sets on the GPU."""
 

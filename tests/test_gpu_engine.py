@@ -184,3 +184,83 @@ def test_server_budget_bounds_resident_state():
     with pytest.raises(N.LdpcpError):
         srv.submit(sweep_set(model=0, n_traj=10 ** 6, max_steps=1000, **FAST))
     srv.close()
+
+
+def test_ragged_batch_fp32_statistics_match_fp64():
+    """A ragged batch of sets of every kind (single / double barrier, under /
+    overdamped, Eb in [2, 6.5], tau over five decades, gamma over two): per set,
+    the FP32 engine's absorbed fraction and MFPT agree with the FP64 engine's
+    within 5 standard errors (same noise, so usually far closer)."""
+    import math
+
+    from langevindynamicsforpcp_amd.sweep import ragged_sets
+    sets = ragged_sets(seed=7, n_sets=24, n_lo=256, n_hi=4096, max_steps=60000, timestep=1e-3)
+    print("[gpu engine] ragged fp32 vs fp64", flush=True)
+    res = {}
+    for prec in (32, 64):
+        sw = Sweep(sets, precision=prec)
+        sw.tune()
+        sw.run()
+        res[prec] = [sw.stats(i) for i in range(len(sets))]
+        sw.close()
+    checked = 0
+    for s, a, b in zip(sets, res[32], res[64]):
+        assert a.n_finished == b.n_finished == s.n_traj
+        n = s.n_traj
+        pb = b.n_absorbed / n
+        se = math.sqrt(max(pb * (1 - pb), 1.0 / n) / n)
+        assert abs(a.n_absorbed / n - pb) <= 5 * se + 1e-12
+        if a.n_absorbed >= 200 and b.n_absorbed >= 200:
+            ma, mb = a.fpt_sum / a.n_absorbed, b.fpt_sum / b.n_absorbed
+            sd = math.sqrt(max(b.fpt_sum2 / b.n_absorbed - mb * mb, 0.0) / b.n_absorbed)
+            assert abs(ma - mb) <= 5 * sd + 1e-9 * mb
+            checked += 1
+    assert checked >= 4
+
+
+def test_reference_case_c1_matches_oracle(oracle):
+    """The reference's CPU-sized case: 32 trajectories (4 ranks x 8 lanes) of
+    the single-barrier underdamped model (barrier 7.2 / 2.0, L = 5.8,
+    w = 0.65, dt = 1e-4, tau = 1e-5), at rest in the left well centre, run
+    to first passage over the barrier or 1e7 steps, Philox seed 1, FP64
+    engine.  Absorption step and TPT start of every trajectory equal the
+    oracle's; the 256-bin FPT histogram holds every FPT in its bin; MSD at the
+    lags up to 1e5 steps equals the oracle's frames."""
+    cap, frames_cap = 10 ** 7, 100001
+    s = sweep_set(model=0, overdamped=False, eb=2.0, ratio=3.6, n_traj=32, max_steps=cap, seed=1, x0=-2.9,
+                  periodic_length=5.8, half_width=(0.65, 0.65), timestep=1e-4, tau=1e-5, randomize_v=False, v0=0.0)
+    print("[gpu engine] reference case C1", flush=True)
+    (paths, st), = run_sweep([s], precision=64)
+    ab, tp = paths[0], paths[1]
+    p = ol.make_params(model=0, overdamped=0, x0=-2.9, v0=0.0, randomize_v=0, periodic_length=5.8,
+                       heights=[[7.2, 2.0], [7.2, 2.0]], half_width=[0.65, 0.65], timestep=1e-4, nsteps=cap,
+                       save_freq=1, tau=1e-5, b_waiting=0, b_fpt=0, b_tpt=0, b_fptcg=0, absorb=1)
+    lags = np.array(st.msd_step, dtype=np.int64)
+    use = (lags > 0) & (lags < frames_cap)
+    want = np.zeros(64)
+    want_n = np.zeros(64, np.int64)
+    fpts = []
+    for t in range(32):
+        out = ol.Lane()
+        fr = np.zeros(frames_cap)
+        oracle.ldo_lane_run_philox(C.byref(p), C.byref(out), 1, t, 0, 0, ol.dptr(fr), frames_cap)
+        assert out.absorbed_step == ab[t], t
+        if ab[t]:
+            assert out.tpt_start_step == tp[t], t
+            fpts.append(ab[t] * 1e-4)
+        for k in np.nonzero(use)[0]:
+            if out.absorbed_step == 0 or out.absorbed_step >= lags[k]:
+                want[k] += (fr[lags[k]] - s.x0) ** 2
+                want_n[k] += 1
+    assert st.n_absorbed == len(fpts) and st.n_absorbed + st.n_capped == 32
+    assert st.fpt_sum == pytest.approx(sum(fpts), rel=1e-12)
+    assert list(np.array(st.msd_n)[use]) == list(want_n[use])
+    np.testing.assert_allclose(np.array(st.msd_sum)[use], want[use], rtol=1e-12)
+    # histogram: log10 bins over [log10 dt, log10 (dt cap)]
+    edges = np.linspace(st.hist_lo, st.hist_hi, 257)
+    hist = np.array(st.hist)
+    assert hist.sum() == len(fpts)
+    for f in fpts:
+        b = int(np.searchsorted(edges, np.log10(f), side="right") - 1)
+        near = [k for k in range(max(b - 1, 0), min(b + 2, 256)) if hist[k] > 0]
+        assert near, f
