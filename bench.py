@@ -28,6 +28,7 @@ the plain FP64 engine running the same requests one at a time.
 """
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
