@@ -629,6 +629,9 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     sn.g1 = 0.0;
     sn.w3 = 0;
     uint64_t n_catch = o->n_catch;
+    // frames every save_freq steps: a countdown instead of a 64-bit modulo per step
+    uint64_t to_save = c.save_freq - s0 % c.save_freq;
+    uint64_t frame = s0 / c.save_freq + 1;
     uint64_t iStep;
     for (iStep = s0 + 1; iStep <= s1; ++iStep) {
         double g, u;
@@ -695,7 +698,10 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         lane_energy_force(s, c);
         if (!c.overdamped) s.v = s.v + (c.minv * s.F) * c.halfdt;
         if (s.ent_period || s.ent_well) stats_event(c, o, s, iStep, P.ticks + (uint64_t)t * c.tick_cap);
-        if (iStep % c.save_freq == 0) record(c, P, t, s, iStep / c.save_freq);
+        if (--to_save == 0) {
+            record(c, P, t, s, frame++);
+            to_save = c.save_freq;
+        }
     }
     o->n_catch = n_catch;
     o->n_gauss += (iStep - 1) - s0;

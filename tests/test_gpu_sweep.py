@@ -139,3 +139,45 @@ def test_sweep_is_independent_of_batching():
     assert np.array_equal(ab1, ab2) and np.array_equal(tp1, tp2)
     alone.close()
     mixed.close()
+
+
+def test_fp32_distributions_match_fp64():
+    """Free-running statistics of the FP32 engine against the all-FP64 engine
+    on the same noise: the MFPT and the mean TPT lie within the FP64 95 %
+    confidence interval, a two-sample Kolmogorov-Smirnov test on the FPT
+    distributions gives p > 0.01, and the MSD agrees within 3 sigma at every
+    lag that both sampled."""
+    from scipy.stats import ks_2samp
+    n = 16384
+    s = sweep_set(model=0, overdamped=True, eb=1.0, ratio=3.0, n_traj=n, max_steps=400000, seed=12, **FAST)
+    print("[gpu sweep] fp32 vs fp64 distributions", flush=True)
+    out = {}
+    for prec in (64, 32):
+        sw = Sweep([s], precision=prec)
+        sw.run()
+        out[prec] = (sw.stats(0), sw.paths(0))
+        sw.close()
+    st64, p64 = out[64]
+    st32, p32 = out[32]
+    assert st64.n_capped == 0 and st32.n_capped == 0
+    m64, m32 = st64.fpt_sum / n, st32.fpt_sum / n
+    se64 = math.sqrt(max(st64.fpt_sum2 / n - m64 * m64, 0.0) / n)
+    assert abs(m32 - m64) < 1.96 * se64
+    t64, t32 = st64.tpt_sum / n, st32.tpt_sum / n
+    tse64 = math.sqrt(max(st64.tpt_sum2 / n - t64 * t64, 0.0) / n)
+    assert abs(t32 - t64) < 1.96 * tse64
+    f64 = p64[0].astype(float) * s.timestep
+    f32 = p32[0].astype(float) * s.timestep
+    assert ks_2samp(f32, f64).pvalue > 0.01
+    checked = 0
+    for k in range(64):
+        na, nb = st32.msd_n[k], st64.msd_n[k]
+        if na < 100 or nb < 100:
+            continue
+        ma, mb = st32.msd_sum[k] / na, st64.msd_sum[k] / nb
+        # standard error of the mean of d^2 for a zero-mean Gaussian displacement
+        # d: Var[d^2] = 2 <d^2>^2
+        sig = math.sqrt(2.0) * mb / math.sqrt(nb)
+        assert abs(ma - mb) <= 3 * sig + 1e-12, k
+        checked += 1
+    assert checked >= 20
