@@ -172,3 +172,97 @@ def test_header_layout_matches_reference(tmp_path):
             assert number.sub("N", a.split("=")[0]) == number.sub("N", b.split("=")[0])
         else:
             assert a == b
+
+
+DOUBLE_INP = """\
+OUTPUTBASE           dbl
+OVERDAMPED           {over}
+X0                   -2.9
+V0                   1.0
+GAMMA                1.0
+KB                   1.0
+TEMPERATURE          1.0
+MASS                 1.0
+RANDOMIZEV           true
+PERIODICLENGTH       5.8
+BARRIERHEIGHTS0      1.0 0.5
+BARRIERHEIGHTS1      2.0 0.5
+BARRIERHALFWIDTH0    0.70
+BARRIERHALFWIDTH1    0.65
+BARRIERPOSITION0     -1.42
+BARRIERPOSITION1     -0.56
+TIMESTEP             0.002
+NUMBEROFSTEPS        200000
+NUMBEROFPREVIOUSSTEPS 0
+NUMBEROFRUNS         1
+FREQSAVE             100
+FREQSTAT             1
+DISTRIBUTIONV0       false
+DISTRIBUTIONS0       true
+TAU                  0.001
+INITIALSTATE         0
+RANDOMIZESTATES      true
+UNIFORMINITALSTATES  false
+EQ                   true 1.0
+WAITINGTIME          true
+FPT                  true
+TPT                  true
+FPTCG                true 2
+FPTCG                true 3
+READRESTART          false
+WRITERESTART         false
+INPUTRESTART         none
+OUTPUTRESTART        none
+WRITEX               false
+NUMBEROFLOGSAMPLES   20
+RANDOMSEED           77
+PRINTTIMER           false
+"""
+
+
+@pytest.mark.parametrize("over", ["true", "false"])
+def test_double_barrier_executable_matches_oracle(oracle, tmp_path, over):
+    """langevinDynamicsDoubleBarrier end to end: FPT2 and FPTCG per-lane means
+    (the statistics the double-barrier code keeps), FPT1 / TPT outputs zero,
+    state counts and MSD against the oracle with the same Philox keys."""
+    (tmp_path / "run.inp").write_text(DOUBLE_INP.format(over=over))
+    ranks, W = 2, 8
+    print(f"[gpu exe] double over={over}", flush=True)
+    r = subprocess.run([os.path.join(BIN, "langevinDynamicsDoubleBarrier"), "-i", "run.inp", "--ranks", str(ranks),
+                        "--simd-width", str(W)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    p = ol.make_params(model=1, overdamped=1 if over == "true" else 0, x0=-2.9, periodic_length=5.8,
+                       heights=[[1.0, 0.5], [2.0, 0.5]], half_width=[0.70, 0.65], position=[-1.42, -0.56],
+                       timestep=2e-3, tau=1e-3, nsteps=200000, save_freq=100, randomize_v=1, cg=[2, 3])
+    seeds = rank_seeds(77, ranks)
+    lanes = []
+    for rk in range(ranks):
+        lanes += oracle_lanes(oracle, p, seeds[rk], 1, W)
+    mean = lambda s, n: s / n if n else 0.0  # noqa: E731
+    assert sum(L.fpt2_n for L, _ in lanes) > 0, "no FPT2 events: the case does not exercise the statistics"
+    exp = {
+        "firstPassageTime2.dat": [mean(L.fpt2_sum, L.fpt2_n) for L, _ in lanes],
+        "firstPassageTimeCG2_0000_0002.dat": [mean(L.cg_sum[0], L.cg_n[0]) for L, _ in lanes],
+        "firstPassageTimeCG2_0001_0003.dat": [mean(L.cg_sum[1], L.cg_n[1]) for L, _ in lanes],
+    }
+    for name, want in exp.items():
+        got = np.fromfile(tmp_path / f"dbl_{name}", "<f8")
+        np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12, err_msg=name)
+    for name in ("firstPassageTime.dat", "transitionPathTime.dat", "transitionPathTime2.dat"):
+        got = np.fromfile(tmp_path / f"dbl_{name}", "<f8")
+        assert got.shape == (ranks * W,) and not got.any(), name
+    s0 = sum(L.state_count[0] for L, _ in lanes)
+    s1 = sum(L.state_count[1] for L, _ in lanes)
+    sc = [ln for ln in (tmp_path / "dbl_stateCount.log").read_text().splitlines() if ln.startswith("# state")]
+    assert sc[0].split()[3] == f"{s0:.0f}" and sc[1].split()[3] == f"{s1:.0f}"
+    lags = np.zeros(512, np.uint64)
+    nl = oracle.ldo_log10_scale(0.2, 2001, 1000, 20, 1, lags.ctypes.data_as(C.POINTER(C.c_uint64)), 512)
+    lags = lags[:nl]
+    total = np.zeros(nl)
+    out = np.zeros(nl)
+    for _, fr in lanes:
+        oracle.ldo_msd_brute(ol.dptr(fr), len(fr), lags.ctypes.data_as(C.POINTER(C.c_uint64)), nl, ol.dptr(out))
+        total += out
+    msd = np.loadtxt(tmp_path / "dbl_msd.log", comments="#")
+    np.testing.assert_allclose(msd[:, 0], 0.2 * lags, rtol=1e-6)
+    np.testing.assert_allclose(msd[:, 1], total / len(lanes), rtol=1e-6, atol=6e-8)
