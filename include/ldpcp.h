@@ -123,6 +123,55 @@ int ldpcp_run_ticks(ldpcp_run* h, uint32_t begin, uint32_t n, double* out);
  * frames), in the reference's summation order. */
 int ldpcp_run_msd(ldpcp_run* h, const uint64_t* lags, uint32_t nlags, double* out);
 
+/* Replaces UserData::m_bInitialValue (langevinDynamicsSingleBarrier.h:1378-1380,
+ * used by BarrierStatus::Initialize :80-84): with UNIFORMINITALSTATES, barriers
+ * created during the run take this state per trajectory ([n_traj], 0 or 1).  It
+ * is 0 for a new handle; the reference's runner carries each lane's initial
+ * state into the next trial, and a restart carries it from the file. */
+int ldpcp_run_set_barrier_default(ldpcp_run* h, const double* state);
+
+/* Checkpoints.  Replace TrajectoryData::WriteRestartFile / ReadRestartFile
+ * (langevinDynamicsSingleBarrier.h:385-576, 577-820; the file-level
+ * WriteRestartFile / ReadRestartFile :1048-1206 and the resumed Integrate
+ * :1397-1400, 1420) with an image of the handle's device state.  The image is
+ * ldpcp_checkpoint_header followed by
+ *   ldpcp_lane     lanes[n_traj]                      (what ldpcp_run_lanes returns)
+ *   int32_t        window_base[n_traj]                (padded to 8 bytes)
+ *   uint64_t       window[n_traj][2][barrier_window]  (bit 63 created, bit 62 state,
+ *                                                      bits 0-61 step of the last hand-over)
+ *   double         frames[n_fields][n_frames][n_traj] (x; with WRITEX v, f, pe, ke, state, barrier)
+ *   double         ticks[n_traj][tick_capacity]
+ * Event lists are held as running sums and counts (ldpcp_lane), and the noise
+ * position is the step count itself (the Philox counter), so a resumed run
+ * continues bit for bit: NSTEPS = a + b in one run equals a run of a steps,
+ * a checkpoint, and a resumed run of b steps. */
+#define LDPCP_CHECKPOINT_MAGIC 0x314B43504350444Cull /* "LDPCPCK1" */
+typedef struct ldpcp_checkpoint_header {
+    uint64_t magic;
+    uint32_t header_bytes, lane_bytes;  /* sizeof of the two structs, as written */
+    uint32_t n_traj, traj_offset;
+    int32_t model, overdamped;
+    int32_t barrier_window, n_fields;
+    uint64_t seed;
+    uint64_t steps_done;                /* absolute step of the state */
+    uint64_t n_frames;                  /* steps_done / save_freq + 1 */
+    uint64_t save_freq;
+    uint64_t tick_capacity;
+    double timestep;
+    uint64_t bytes;                     /* size of the whole image */
+} ldpcp_checkpoint_header;
+
+/* Size of the image ldpcp_run_checkpoint writes for this handle now. */
+int ldpcp_run_checkpoint_size(ldpcp_run* h, uint64_t* bytes);
+/* Writes the image after ldpcp_run_integrate (or after ldpcp_run_resume). */
+int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes);
+/* Loads an image into a handle created with the same parameters, trajectory
+ * count and offset (NSTEPS may differ: it is the number of further steps).  The
+ * next ldpcp_run_integrate continues from the image's step, appending frames to
+ * the image's; ldpcp_run_n_frames then counts both.  Such a handle integrates
+ * once. */
+int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes);
+
 /* Replaces SAFE_DELETE(leOMP) (langevinDynamicsSingleBarrier.h:3972). */
 int ldpcp_run_destroy(ldpcp_run* h);
 

@@ -110,6 +110,17 @@ class ServerConfig(C.Structure):
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
+CHECKPOINT_MAGIC = 0x314B43504350444C  # "LDPCPCK1"
+
+
+class CheckpointHeader(C.Structure):
+    _fields_ = [("magic", C.c_uint64), ("header_bytes", C.c_uint32), ("lane_bytes", C.c_uint32),
+                ("n_traj", C.c_uint32), ("traj_offset", C.c_uint32), ("model", C.c_int32),
+                ("overdamped", C.c_int32), ("barrier_window", C.c_int32), ("n_fields", C.c_int32),
+                ("seed", C.c_uint64), ("steps_done", C.c_uint64), ("n_frames", C.c_uint64),
+                ("save_freq", C.c_uint64), ("tick_capacity", C.c_uint64), ("timestep", C.c_double),
+                ("bytes", C.c_uint64)]
+
 # (name, restype, argtypes) for every entry point of include/ldpcp.h
 _P = C.c_void_p
 _SIGNATURES = [
@@ -123,6 +134,10 @@ _SIGNATURES = [
     ("ldpcp_run_frames", C.c_int, [_P, C.c_int32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
     ("ldpcp_run_ticks", C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
     ("ldpcp_run_msd", C.c_int, [_P, C.POINTER(C.c_uint64), C.c_uint32, C.POINTER(C.c_double)]),
+    ("ldpcp_run_set_barrier_default", C.c_int, [_P, C.POINTER(C.c_double)]),
+    ("ldpcp_run_checkpoint_size", C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    ("ldpcp_run_checkpoint", C.c_int, [_P, C.c_void_p, C.c_uint64]),
+    ("ldpcp_run_resume", C.c_int, [_P, C.c_void_p, C.c_uint64]),
     ("ldpcp_run_destroy", C.c_int, [_P]),
     ("ldpcp_sweep_create", C.c_int, [C.POINTER(SweepSet), C.c_int32, C.c_int32, C.POINTER(_P)]),
     ("ldpcp_sweep_launch", C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),

@@ -32,7 +32,7 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contr
              "-Wall", "-Wno-unused-function", "-I", os.path.join(REPO, "include")]
 
 LIB_SOURCES = ["ldpcp_common.cpp", "ldpcp_run.hip", "ldpcp_engine.hip", "ldpcp_server.hip"]
-HOST_COMMON = ["input_parser.cpp", "runner.cpp", "text_util.cpp"]
+HOST_COMMON = ["input_parser.cpp", "restart_file.cpp", "runner.cpp", "text_util.cpp"]
 EXECUTABLES = {"langevinDynamicsSingleBarrier": ("main_single.cpp", 0),
                "langevinDynamicsDoubleBarrier": ("main_double.cpp", 1)}
 

@@ -55,6 +55,7 @@ struct RunPtrs {
     const double* c;     // replay catch-up uniforms [traj][cap]
     const int8_t* binit; // replay barrier states [array][traj][bn]
     double* ticks;       // barrier ticks [traj][tick_cap]
+    double* bdefault;    // state of barriers created with UNIFORMINITALSTATES [traj]
 };
 
 // Constants of IntegrateInitializeVariables (langevinDynamicsSingleBarrier.h:1520-1618,
@@ -312,7 +313,10 @@ __device__ __forceinline__ void step_noise(const RunConst& c, const RunPtrs& P, 
 __device__ __forceinline__ int barrier_init_state(const RunConst& c, const RunPtrs& P, uint32_t t, int array, int id,
                                                   int& err)
 {
-    if (c.same_init) return 0;  // UserData::m_bInitialValue of a fresh run (Single.h:1378-1380)
+    // UserData::m_bInitialValue: the lane's BarrierState0Instance as it stood before step 0
+    // (Single.h:1378-1380, 80-84): 0 for a fresh integrator, the previous trial's or the
+    // restart file's initial state otherwise
+    if (c.same_init) return (int)P.bdefault[t];
     if (c.replay) {
         const int k = id - c.bmin;
         if (k < 0 || k >= c.bn) {
@@ -742,7 +746,10 @@ struct ldpcp_run {
     ldpcp::RunPtrs P;
     hipStream_t stream;
     void* state_block;
+    void* frames_block;   // frames of a resumed run (prior frames + this run's), else null
     uint64_t n_frames;
+    uint64_t step_base;   // absolute step the next integrate starts after
+    int phase;            // 0 fresh, 1 resumed (next integrate continues), 2 resumed run done
     bool integrated;
 };
 
@@ -805,7 +812,9 @@ extern "C" int ldpcp_run_create(const ldpcp_run_params* p, uint32_t n_traj, uint
     const uint64_t bytes_wbase = n * sizeof(int32_t);
     const uint64_t bytes_frames = fields * h->c.frames_cap * n * sizeof(double);
     const uint64_t bytes_ticks = n * p->tick_capacity * sizeof(double);
-    const uint64_t total = bytes_soa + bytes_lanes + bytes_win + bytes_wbase + bytes_frames + bytes_ticks + 1024;
+    const uint64_t bytes_bdef = n * sizeof(double);
+    const uint64_t total =
+        bytes_soa + bytes_lanes + bytes_win + bytes_wbase + bytes_frames + bytes_ticks + bytes_bdef + 1024;
     hipError_t e = hipMalloc(&h->state_block, total);
     if (e != hipSuccess) {
         set_error("ldpcp_run_create: hipMalloc(%llu bytes) failed: %s", (unsigned long long)total, hipGetErrorString(e));
@@ -833,11 +842,21 @@ extern "C" int ldpcp_run_create(const ldpcp_run_params* p, uint32_t n_traj, uint
     h->P.frames = (double*)b;
     b += bytes_frames;
     h->P.ticks = (double*)b;
+    b += bytes_ticks;
+    h->P.bdefault = (double*)b;
     h->P.g = h->P.f = h->P.c = nullptr;
     h->P.binit = nullptr;
     e = hipStreamCreate(&h->stream);
     if (e != hipSuccess) {
         set_error("ldpcp_run_create: hipStreamCreate failed: %s", hipGetErrorString(e));
+        (void)hipFree(h->state_block);
+        delete h;
+        return LDPCP_EDEVICE;
+    }
+    e = hipMemsetAsync(h->P.bdefault, 0, bytes_bdef, h->stream);
+    if (e != hipSuccess) {
+        set_error("ldpcp_run_create: hipMemsetAsync failed: %s", hipGetErrorString(e));
+        (void)hipStreamDestroy(h->stream);
         (void)hipFree(h->state_block);
         delete h;
         return LDPCP_EDEVICE;
@@ -855,8 +874,9 @@ extern "C" int ldpcp_run_set_replay(ldpcp_run* h, const double* gauss, uint64_t 
         return LDPCP_EINVAL;
     }
     const uint64_t n = h->c.n_traj;
-    const uint64_t need_g = h->c.init_gauss + h->c.nsteps_eff;
-    const uint64_t need_f = h->c.init_flip + (h->c.tau_zero ? 0 : h->c.nsteps_eff);
+    const uint64_t end = h->step_base + h->c.nsteps_eff;  // replay streams are indexed by absolute step
+    const uint64_t need_g = h->c.init_gauss + end;
+    const uint64_t need_f = h->c.init_flip + (h->c.tau_zero ? 0 : end);
     if (n_gauss < need_g || n_flip < need_f) {
         set_error("ldpcp_run_set_replay: streams too short (gauss %llu < %llu or flip %llu < %llu)",
                   (unsigned long long)n_gauss, (unsigned long long)need_g, (unsigned long long)n_flip,
@@ -898,22 +918,32 @@ extern "C" int ldpcp_run_integrate(ldpcp_run* h, uint64_t steps_per_launch)
         set_error("ldpcp_run_integrate: null handle");
         return LDPCP_EINVAL;
     }
+    if (h->phase == 2) {
+        set_error("ldpcp_run_integrate: this handle already continued a checkpoint; create a new handle");
+        return LDPCP_EINVAL;
+    }
     if (steps_per_launch == 0) steps_per_launch = 1ull << 20;
     const uint64_t n = h->c.n_traj;
-    LDPCP_HIP_TRY(hipMemsetAsync(h->P.lanes, 0, n * sizeof(ldpcp_lane), h->stream));
-    LDPCP_HIP_TRY(hipMemsetAsync(h->P.win, 0, n * 2 * (uint64_t)h->c.wb * sizeof(uint64_t), h->stream));
-    const uint64_t fields = h->params.save_all ? kFields : 1;
-    LDPCP_HIP_TRY(hipMemsetAsync(h->P.frames, 0, fields * h->c.frames_cap * n * sizeof(double), h->stream));
     const dim3 block(256), grid((unsigned)((n + 255) / 256));
-    hipLaunchKernelGGL(run_init_kernel, grid, block, 0, h->stream, h->c, h->P);
-    LDPCP_HIP_TRY(hipGetLastError());
-    for (uint64_t s0 = 0; s0 < h->c.nsteps_eff; s0 += steps_per_launch) {
-        const uint64_t s1 = (s0 + steps_per_launch < h->c.nsteps_eff) ? s0 + steps_per_launch : h->c.nsteps_eff;
+    if (h->phase == 0) {
+        LDPCP_HIP_TRY(hipMemsetAsync(h->P.lanes, 0, n * sizeof(ldpcp_lane), h->stream));
+        LDPCP_HIP_TRY(hipMemsetAsync(h->P.win, 0, n * 2 * (uint64_t)h->c.wb * sizeof(uint64_t), h->stream));
+        const uint64_t fields = h->params.save_all ? kFields : 1;
+        LDPCP_HIP_TRY(hipMemsetAsync(h->P.frames, 0, fields * h->c.frames_cap * n * sizeof(double), h->stream));
+        hipLaunchKernelGGL(run_init_kernel, grid, block, 0, h->stream, h->c, h->P);
+        LDPCP_HIP_TRY(hipGetLastError());
+    }
+    // steps (step_base, step_base + NSTEPS]: a resumed run continues the absolute step
+    // count (Single.h:1420, 1471), which is also the Philox counter of its noise
+    const uint64_t end = h->step_base + h->c.nsteps_eff;
+    for (uint64_t s0 = h->step_base; s0 < end; s0 += steps_per_launch) {
+        const uint64_t s1 = (s0 + steps_per_launch < end) ? s0 + steps_per_launch : end;
         hipLaunchKernelGGL(run_step_kernel, grid, block, 0, h->stream, h->c, h->P, s0, s1);
         LDPCP_HIP_TRY(hipGetLastError());
     }
     LDPCP_HIP_TRY(hipStreamSynchronize(h->stream));
     h->integrated = true;
+    if (h->phase == 1) h->phase = 2;
     // surface the first per-lane error
     std::vector<ldpcp_lane> lanes(n);
     LDPCP_HIP_TRY(hipMemcpy(lanes.data(), h->P.lanes, n * sizeof(ldpcp_lane), hipMemcpyDeviceToHost));
@@ -1011,10 +1041,243 @@ extern "C" int ldpcp_run_msd(ldpcp_run* h, const uint64_t* lags, uint32_t nlags,
     return LDPCP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Checkpoints: the device state of a handle as one self-describing image
+// (the state TrajectoryData::WriteRestartFile saves, langevinDynamicsSingleBarrier.h:385-576,
+// with running sums in place of event lists and Philox counters in place of MKL bank
+// positions).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct CkptLayout {
+    uint64_t lanes, wbase, win, frames, ticks, total;
+};
+
+CkptLayout ckpt_layout(uint64_t n, uint64_t wb, uint64_t fields, uint64_t n_frames, uint64_t tick_cap)
+{
+    CkptLayout L;
+    uint64_t o = sizeof(ldpcp_checkpoint_header);
+    L.lanes = o;
+    o += n * sizeof(ldpcp_lane);
+    L.wbase = o;
+    o += (n * sizeof(int32_t) + 7) & ~7ull;
+    L.win = o;
+    o += n * 2 * wb * sizeof(uint64_t);
+    L.frames = o;
+    o += fields * n_frames * n * sizeof(double);
+    L.ticks = o;
+    o += n * tick_cap * sizeof(double);
+    L.total = o;
+    return L;
+}
+
+uint64_t run_fields(const ldpcp_run* h) { return h->params.save_all ? kFields : 1; }
+
+// absolute step of the state a checkpoint would hold
+uint64_t run_steps_done(const ldpcp_run* h) { return h->phase == 1 ? h->step_base : h->step_base + h->c.nsteps_eff; }
+
+CkptLayout run_layout(const ldpcp_run* h)
+{
+    return ckpt_layout(h->c.n_traj, (uint64_t)h->c.wb, run_fields(h), run_steps_done(h) / h->c.save_freq + 1,
+                       h->c.tick_cap);
+}
+
+}  // namespace
+
+extern "C" int ldpcp_run_set_barrier_default(ldpcp_run* h, const double* state)
+{
+    if (!h || !state) {
+        set_error("ldpcp_run_set_barrier_default: null argument");
+        return LDPCP_EINVAL;
+    }
+    for (uint32_t t = 0; t < h->c.n_traj; ++t)
+        if (state[t] != 0.0 && state[t] != 1.0) {
+            set_error("ldpcp_run_set_barrier_default: trajectory %u: state must be 0 or 1", t);
+            return LDPCP_EINVAL;
+        }
+    LDPCP_HIP_TRY(hipMemcpy(h->P.bdefault, state, h->c.n_traj * sizeof(double), hipMemcpyHostToDevice));
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_checkpoint_size(ldpcp_run* h, uint64_t* bytes)
+{
+    if (!h || !bytes) {
+        set_error("ldpcp_run_checkpoint_size: null argument");
+        return LDPCP_EINVAL;
+    }
+    *bytes = run_layout(h).total;
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes)
+{
+    if (!h || !buf) {
+        set_error("ldpcp_run_checkpoint: null argument");
+        return LDPCP_EINVAL;
+    }
+    if (!h->integrated && h->phase != 1) {
+        set_error("ldpcp_run_checkpoint: the handle holds no state yet (integrate or resume first)");
+        return LDPCP_EINVAL;
+    }
+    const uint64_t n = h->c.n_traj, wb = (uint64_t)h->c.wb, fields = run_fields(h);
+    const CkptLayout L = run_layout(h);
+    if (bytes < L.total) {
+        set_error("ldpcp_run_checkpoint: buffer of %llu bytes, need %llu", (unsigned long long)bytes,
+                  (unsigned long long)L.total);
+        return LDPCP_ERANGE;
+    }
+    char* b = (char*)buf;
+    ldpcp_lane* lanes = (ldpcp_lane*)(b + L.lanes);
+    int rc = ldpcp_run_lanes(h, 0, (uint32_t)n, lanes);
+    if (rc) return rc;
+    for (uint64_t t = 0; t < n; ++t)
+        if (lanes[t].error) {
+            set_error("ldpcp_run_checkpoint: trajectory %llu stopped with error %d; its state is not resumable",
+                      (unsigned long long)t, lanes[t].error);
+            return LDPCP_ERANGE;
+        }
+    ldpcp_checkpoint_header hd;
+    memset(&hd, 0, sizeof(hd));
+    hd.magic = LDPCP_CHECKPOINT_MAGIC;
+    hd.header_bytes = sizeof(ldpcp_checkpoint_header);
+    hd.lane_bytes = sizeof(ldpcp_lane);
+    hd.n_traj = (uint32_t)n;
+    hd.traj_offset = h->c.traj_offset;
+    hd.model = h->params.model;
+    hd.overdamped = h->params.overdamped;
+    hd.barrier_window = h->c.wb;
+    hd.n_fields = (int32_t)fields;
+    hd.seed = h->params.seed;
+    hd.steps_done = run_steps_done(h);
+    hd.n_frames = hd.steps_done / h->c.save_freq + 1;
+    hd.save_freq = h->c.save_freq;
+    hd.tick_capacity = h->c.tick_cap;
+    hd.timestep = h->params.timestep;
+    hd.bytes = L.total;
+    memcpy(b, &hd, sizeof(hd));
+    LDPCP_HIP_TRY(hipMemcpy(b + L.wbase, h->P.wbase, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    LDPCP_HIP_TRY(hipMemcpy(b + L.win, h->P.win, n * 2 * wb * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    // the frames recorded so far: the first n_frames rows of each field's plane
+    for (uint64_t f = 0; f < fields; ++f)
+        LDPCP_HIP_TRY(hipMemcpy(b + L.frames + f * hd.n_frames * n * sizeof(double),
+                                h->P.frames + f * h->c.frames_cap * n, hd.n_frames * n * sizeof(double),
+                                hipMemcpyDeviceToHost));
+    if (h->c.tick_cap)
+        LDPCP_HIP_TRY(hipMemcpy(b + L.ticks, h->P.ticks, n * h->c.tick_cap * sizeof(double), hipMemcpyDeviceToHost));
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes)
+{
+    if (!h || !buf || bytes < sizeof(ldpcp_checkpoint_header)) {
+        set_error("ldpcp_run_resume: null argument or truncated image");
+        return LDPCP_EINVAL;
+    }
+    ldpcp_checkpoint_header hd;
+    memcpy(&hd, buf, sizeof(hd));
+    if (hd.magic != LDPCP_CHECKPOINT_MAGIC || hd.header_bytes != sizeof(ldpcp_checkpoint_header) ||
+        hd.lane_bytes != sizeof(ldpcp_lane)) {
+        set_error("ldpcp_run_resume: not a checkpoint image of this library version");
+        return LDPCP_EINVAL;
+    }
+    const uint64_t n = h->c.n_traj, wb = (uint64_t)h->c.wb, fields = run_fields(h);
+    struct Check {
+        bool ok;
+        const char* what;
+    } checks[] = {
+        {hd.n_traj == n, "trajectory count"},
+        {hd.traj_offset == h->c.traj_offset, "trajectory offset"},
+        {hd.model == h->params.model, "model"},
+        {hd.overdamped == h->params.overdamped, "overdamped flag"},
+        {hd.barrier_window == h->c.wb, "barrier window"},
+        {(uint64_t)hd.n_fields == fields, "saved fields (WRITEX)"},
+        {hd.seed == h->params.seed, "seed"},
+        {hd.save_freq == h->c.save_freq, "save frequency"},
+        {hd.timestep == h->params.timestep, "time step"},
+        {hd.n_frames == hd.steps_done / hd.save_freq + 1, "frame count"},
+    };
+    for (const Check& k : checks)
+        if (!k.ok) {
+            set_error("ldpcp_run_resume: inconsistent %s", k.what);
+            return LDPCP_EINVAL;
+        }
+    const CkptLayout L = ckpt_layout(n, wb, fields, hd.n_frames, hd.tick_capacity);
+    if (hd.bytes != L.total || bytes < L.total) {
+        set_error("ldpcp_run_resume: image of %llu bytes, its header describes %llu", (unsigned long long)bytes,
+                  (unsigned long long)L.total);
+        return LDPCP_EINVAL;
+    }
+    if (h->c.replay) {
+        const uint64_t end = hd.steps_done + h->c.nsteps_eff;
+        if (h->c.n_gauss < h->c.init_gauss + end || h->c.n_flip < h->c.init_flip + (h->c.tau_zero ? 0 : end)) {
+            set_error("ldpcp_run_resume: replay streams end before step %llu", (unsigned long long)end);
+            return LDPCP_ERANGE;
+        }
+    }
+    const char* b = (const char*)buf;
+    const ldpcp_lane* lanes = (const ldpcp_lane*)(b + L.lanes);
+    for (uint64_t t = 0; t < n; ++t)
+        if (lanes[t].error || lanes[t].steps_done != hd.steps_done) {
+            set_error("ldpcp_run_resume: trajectory %llu is not at step %llu", (unsigned long long)t,
+                      (unsigned long long)hd.steps_done);
+            return LDPCP_EINVAL;
+        }
+    // frame buffer for the prior frames and this run's
+    const uint64_t cap = (hd.steps_done + h->c.nsteps_eff) / h->c.save_freq + 1;
+    void* fb = nullptr;
+    LDPCP_HIP_TRY(hipMalloc(&fb, fields * cap * n * sizeof(double)));
+    if (h->frames_block) (void)hipFree(h->frames_block);
+    h->frames_block = fb;
+    h->P.frames = (double*)fb;
+    h->c.frames_cap = cap;
+    h->n_frames = cap;
+    LDPCP_HIP_TRY(hipMemsetAsync(fb, 0, fields * cap * n * sizeof(double), h->stream));
+    LDPCP_HIP_TRY(hipStreamSynchronize(h->stream));
+    for (uint64_t f = 0; f < fields; ++f)
+        LDPCP_HIP_TRY(hipMemcpy(h->P.frames + f * cap * n, b + L.frames + f * hd.n_frames * n * sizeof(double),
+                                hd.n_frames * n * sizeof(double), hipMemcpyHostToDevice));
+    // per-trajectory state: the statistics record as is, the stepping state into its columns
+    LDPCP_HIP_TRY(hipMemcpy(h->P.lanes, lanes, n * sizeof(ldpcp_lane), hipMemcpyHostToDevice));
+    std::vector<double> col(n);
+    struct Col {
+        double* dst;
+        size_t off;
+    } cols[] = {{h->P.x, offsetof(ldpcp_lane, x)},           {h->P.v, offsetof(ldpcp_lane, v)},
+                {h->P.F, offsetof(ldpcp_lane, f)},           {h->P.E, offsetof(ldpcp_lane, e)},
+                {h->P.R, offsetof(ldpcp_lane, r_prev)},      {h->P.state, offsetof(ldpcp_lane, state)},
+                {h->P.barrier, offsetof(ldpcp_lane, barrier)}, {h->P.well, offsetof(ldpcp_lane, well)},
+                {h->P.periodic, offsetof(ldpcp_lane, periodic)}, {h->P.xp, offsetof(ldpcp_lane, xp)},
+                {h->P.local, offsetof(ldpcp_lane, local_barrier)}, {h->P.bdefault, offsetof(ldpcp_lane, state0_instance)}};
+    for (const Col& c : cols) {
+        for (uint64_t t = 0; t < n; ++t) memcpy(&col[t], (const char*)(lanes + t) + c.off, sizeof(double));
+        LDPCP_HIP_TRY(hipMemcpy(c.dst, col.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    std::vector<uint64_t> cnt(n);
+    for (int k = 0; k < 2; ++k) {
+        for (uint64_t t = 0; t < n; ++t) cnt[t] = (uint64_t)lanes[t].state_count[k];
+        LDPCP_HIP_TRY(hipMemcpy(k ? h->P.cnt1 : h->P.cnt0, cnt.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice));
+    }
+    LDPCP_HIP_TRY(hipMemcpy(h->P.wbase, b + L.wbase, n * sizeof(int32_t), hipMemcpyHostToDevice));
+    LDPCP_HIP_TRY(hipMemcpy(h->P.win, b + L.win, n * 2 * wb * sizeof(uint64_t), hipMemcpyHostToDevice));
+    if (h->c.tick_cap) {
+        // barrier ticks beyond the smaller of the two capacities stay counted but unlisted
+        std::vector<double> tk(n * h->c.tick_cap, 0.0);
+        const uint64_t keep = hd.tick_capacity < h->c.tick_cap ? hd.tick_capacity : h->c.tick_cap;
+        for (uint64_t t = 0; t < n; ++t)
+            memcpy(&tk[t * h->c.tick_cap], b + L.ticks + t * hd.tick_capacity * sizeof(double), keep * sizeof(double));
+        LDPCP_HIP_TRY(hipMemcpy(h->P.ticks, tk.data(), tk.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    h->step_base = hd.steps_done;
+    h->phase = 1;
+    h->integrated = false;
+    return LDPCP_OK;
+}
+
 extern "C" int ldpcp_run_destroy(ldpcp_run* h)
 {
     if (!h) return LDPCP_OK;
     if (h->c.replay) (void)hipFree((void*)h->P.g);
+    if (h->frames_block) (void)hipFree(h->frames_block);
     (void)hipFree(h->state_block);
     (void)hipStreamDestroy(h->stream);
     delete h;

@@ -1,0 +1,39 @@
+// restart_file.h -- the runner's restart files, <OUTPUTRESTART>_%05d_%05d.restart
+// per (rank, trial), as LangevinEquationSIMDMultiple::WriteRestartFile /
+// ReadRestartFile write and check them (langevinDynamicsSingleBarrier.h:1048-1206;
+// langevinDynamicsDoubleBarrier.h:1202-1368).
+//
+// The file keeps the reference's name, its settings header in the same order
+// and encoding (int 4 B, size_t / long long 8 B, double 8 B, little-endian) and
+// its read-side consistency checks with the same messages.  The body after the
+// header is the library's checkpoint image (ldpcp_checkpoint_header + device
+// state, include/ldpcp.h) in place of the reference's per-lane event lists and
+// MKL bank positions, so the signature differs from the reference's 2345: each
+// program rejects the other's file at its first check.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "runner.h"
+
+namespace ldpcp_host {
+
+constexpr int kRestartSignature = 0x4C445043;  // the reference's files carry 2345
+constexpr int kRestartEnd = 9990;
+
+std::string restart_file_name(const std::string& base, int rank, int trial);
+
+// WriteRestartFile: settings header, the three noise positions (here the
+// Philox step counter), the checkpoint image and the end marker.
+bool write_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, uint64_t nsteps_total,
+                        const std::vector<char>& image);
+
+// ReadRestartFile: false (after the reference's message on stderr) when the
+// file is missing or any setting differs from c.  On success image holds the
+// checkpoint image and nsteps_total the header's step count.
+bool read_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, std::vector<char>& image,
+                       uint64_t* nsteps_total);
+
+}  // namespace ldpcp_host

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "ldpcp.h"
+#include "restart_file.h"
 #include "text_util.h"
 
 namespace ldpcp_host {
@@ -272,11 +273,6 @@ int run(const RunnerConfig& cfg)
         fprintf(stderr, "Error: numberOfFrame < 1000\n");
         return 0;
     }
-    if (cfg.read_restart || cfg.nsteps_previous != 0) {
-        fprintf(stderr, "Error: can't read restart file on rank %10d (restart input is not supported by this build)\n",
-                0);
-        return 0;
-    }
     Header hd;
     hd.c = &cfg;
     hd.nrun = cfg.ntrial * W * R;
@@ -296,8 +292,6 @@ int run(const RunnerConfig& cfg)
     } else {
         fprintf(stderr, "Interval = %10zd\n", interval.size());
     }
-    if (cfg.write_restart)
-        fprintf(stderr, "Warning: WRITERESTART is not supported by this build; no restart file is written\n");
 
     const size_t nl = interval.size();
     const size_t ncg = cfg.fptcg ? cfg.cg_size.size() : 0;
@@ -362,8 +356,13 @@ int run(const RunnerConfig& cfg)
     std::vector<ldpcp_lane> lanes(W);
     std::vector<double> msd0((size_t)W * nl);
     std::vector<double> ticks(cfg.waiting ? (size_t)W * tick_cap : 0);
-    const uint64_t nf = cfg.nsteps / cfg.save_freq + 1;
-    std::vector<double> frames(cfg.write_x ? (size_t)7 * W * nf : 0);
+    std::vector<double> frames;
+
+    // each rank's lanes' initial barrier states, carried into the next trial as
+    // UserData::m_bInitialValue (Single.h:1378-1380)
+    std::vector<std::vector<double>> state0_prev(R, std::vector<double>(W, 0.0));
+    std::vector<char> image;
+    uint64_t nsteps_before = 0;  // m_NumberOfStepDonePreviously
 
     for (int i = 0; i < cfg.ntrial; ++i) {
         fprintf(stderr, "##################################################################\n");
@@ -375,12 +374,41 @@ int run(const RunnerConfig& cfg)
             p.seed = (uint64_t)(uint32_t)seeds[r];
             ldpcp_run* h = nullptr;
             if (ldpcp_run_create(&p, (uint32_t)W, (uint32_t)((uint64_t)i * W), &h)) die("ldpcp_run_create");
+            if (cfg.same_initial_barrier && ldpcp_run_set_barrier_default(h, state0_prev[r].data()))
+                die("ldpcp_run_set_barrier_default");
+            nsteps_before = 0;
+            if (cfg.read_restart) {
+                // PreIntegrate -> ReadRestartFile (Single.h:1346-1351); a failure ends the
+                // run as the runner's integrator-failure check does (:3436-3446)
+                bool ok = read_restart_file(cfg, W, r, i, image, &nsteps_before);
+                if (ok && ldpcp_run_resume(h, image.data(), image.size())) {
+                    fprintf(stderr, "Error: %s\n", ldpcp_last_error());
+                    ok = false;
+                }
+                if (!ok) {
+                    fprintf(stderr, "Error: can't read restart file on rank %10d\n", r);
+                    fprintf(stderr, "Error: Integrator failure\n");
+                    ldpcp_run_destroy(h);
+                    exit(0);
+                }
+            }
             if (ldpcp_run_integrate(h, cfg.steps_per_launch)) die("ldpcp_run_integrate");
+            if (cfg.write_restart) {
+                uint64_t bytes = 0;
+                if (ldpcp_run_checkpoint_size(h, &bytes)) die("ldpcp_run_checkpoint_size");
+                image.resize(bytes);
+                if (ldpcp_run_checkpoint(h, image.data(), bytes)) die("ldpcp_run_checkpoint");
+                if (!write_restart_file(cfg, W, r, i, nsteps_before + cfg.nsteps, image))
+                    fprintf(stderr, "Error: can't open restart file on rank %10d\n", r);
+            }
             auto t1 = clock::now();
             t_integrate += std::chrono::duration<double>(t1 - t0).count();
             if (ldpcp_run_lanes(h, 0, (uint32_t)W, lanes.data())) die("ldpcp_run_lanes");
             if (ldpcp_run_msd(h, interval.data(), (uint32_t)nl, msd0.data())) die("ldpcp_run_msd");
             if (cfg.waiting && ldpcp_run_ticks(h, 0, (uint32_t)W, ticks.data())) die("ldpcp_run_ticks");
+            // frames of this run, after a restart's frames when it continues one
+            const uint64_t nf = ldpcp_run_n_frames(h);
+            if (cfg.write_x) frames.resize((size_t)7 * W * nf);
             if (cfg.write_x)
                 for (int field = 0; field < 7; ++field)
                     if (ldpcp_run_frames(h, field, 0, (uint32_t)W, frames.data() + (size_t)field * W * nf))
@@ -407,6 +435,7 @@ int run(const RunnerConfig& cfg)
                     wt_n[r] += (double)L.wt_n;
                     if (L.wt_n > 0) wt_max[r] = std::max(wt_max[r], L.wt_max);
                 }
+                state0_prev[r][ch] = L.state0_instance;
                 if (cfg.v0_distribution) h_v0[r].add(L.v0_instance);
                 if (cfg.state0_distribution) h_s0[r].add(L.state0_instance);
                 if (cfg.write_x) {

@@ -55,6 +55,8 @@ class LangevinEquation:
         self._h = None
         self._seed = 0
         self._replay = None
+        self._resume = None
+        self._bdefault = None
         self._lanes = None
         self._frames = {}
 
@@ -104,6 +106,27 @@ class LangevinEquation:
         self._replay = (np.ascontiguousarray(gauss, np.float64), np.ascontiguousarray(flip, np.float64),
                         np.ascontiguousarray(catchup, np.float64), np.ascontiguousarray(binit, np.int8), int(bmin))
 
+    def SetBarrierDefault(self, states):
+        """UserData::m_bInitialValue (Single.h:1378-1380): the state barriers created
+        under UNIFORMINITALSTATES take, per trajectory (the previous trial's
+        GetBarrierState0Instance in the reference's trial loop)."""
+        self._bdefault = np.ascontiguousarray(states, np.float64)
+
+    def Resume(self, image):
+        """ReadRestartFile (Single.h:1137-1206): the next Integrate continues the
+        checkpointed run for m_NumberOfStep further steps."""
+        self._resume = bytes(image)
+
+    def Checkpoint(self):
+        """The state WriteRestartFile saves (Single.h:1048-1136), as the library's
+        checkpoint image (bytes)."""
+        lib = N.load()
+        n = C.c_uint64()
+        N.check(lib.ldpcp_run_checkpoint_size(self._h, C.byref(n)), "ldpcp_run_checkpoint_size")
+        buf = C.create_string_buffer(n.value)
+        N.check(lib.ldpcp_run_checkpoint(self._h, buf, n.value), "ldpcp_run_checkpoint")
+        return buf.raw
+
     def _create(self):
         lib = N.load()
         if self._h is not None:
@@ -120,6 +143,12 @@ class LangevinEquation:
                 f.ctypes.data_as(C.POINTER(C.c_double)), f.shape[0] if f.size else 0,
                 c.ctypes.data_as(C.POINTER(C.c_double)), c.shape[1] if c.size else 0,
                 b.ctypes.data_as(C.POINTER(C.c_int8)), bmin, b.shape[2] if b.size else 0), "ldpcp_run_set_replay")
+        if self._bdefault is not None:
+            N.check(lib.ldpcp_run_set_barrier_default(h, self._bdefault.ctypes.data_as(C.POINTER(C.c_double))),
+                    "ldpcp_run_set_barrier_default")
+        if self._resume is not None:
+            img = C.create_string_buffer(self._resume, len(self._resume))
+            N.check(lib.ldpcp_run_resume(h, img, len(self._resume)), "ldpcp_run_resume")
 
     def Integrate(self, iTrial=0, bTimer=False):
         """Integrate(iTrial) (Single.h:1386-1519) on the GPU."""

@@ -43,13 +43,12 @@ class _Reader:
         return self.vec(self.u64())
 
 
-def decode(path, overdamped=True, tpt=True, fptcg=True):
-    """Decode one restart file into a dict of header fields and per-lane data."""
-    with open(path, "rb") as f:
-        r = _Reader(f.read())
+def read_header(r, model=0):
+    """The settings header (Single.h:1054-1101; the double barrier writes four
+    heights, the barrier positions and the apex xc, Double.h:1232-1241)."""
     h = {}
     h["signature"] = r.i32()
-    W = h["simd_width"] = r.i32()
+    h["simd_width"] = r.i32()
     for k in ("x0", "v0", "gamma", "kb", "temperature", "mass", "timestep"):
         h[k] = r.f64()
     h["nsteps_total"] = r.u64()
@@ -59,7 +58,11 @@ def decode(path, overdamped=True, tpt=True, fptcg=True):
     h["save_flags"] = [r.i32() for _ in range(4)]
     h["randomize_v"] = r.i32()
     h["randomize_eta"] = r.i32()
-    for k in ("periodic_length", "h0", "h1", "initial_state"):
+    if model == 0:
+        keys = ("periodic_length", "h0", "h1", "initial_state")
+    else:
+        keys = ("periodic_length", "h00", "h01", "h10", "h11", "initial_state", "position0", "position1", "xc")
+    for k in keys:
         h[k] = r.f64()
     h["randomize_state"] = r.i32()
     h["same_initial_barrier"] = r.i32()
@@ -68,6 +71,15 @@ def decode(path, overdamped=True, tpt=True, fptcg=True):
     h["save_flags2"] = [r.i32() for _ in range(3)]
     h["rank"] = r.i32()
     h["rng"] = [(r.i64(), r.u64()) for _ in range(3)]
+    return h
+
+
+def decode(path, overdamped=True, tpt=True, fptcg=True):
+    """Decode one restart file into a dict of header fields and per-lane data."""
+    with open(path, "rb") as f:
+        r = _Reader(f.read())
+    h = read_header(r)
+    W = h["simd_width"]
     assert r.i32() == 9998
     b_wt = r.i32()
     b_fpt = r.i32()

@@ -24,9 +24,12 @@ class NumpyBanks:
             arr[:] = self.rng[which].random(n)
 
 
-def run_rank_recorded(oracle, p, W, seed, bmin=-300, bn=600, cap=None, bank=4096):
+def run_rank_recorded(oracle, p, W, seed, bmin=-300, bn=600, cap=None, bank=4096, trials=1):
     """Run one rank of W lanes; return lanes, frames [frame][lane] and the
-    role-tagged per-lane streams."""
+    role-tagged per-lane streams.  With trials > 1 the rank first runs
+    trials - 1 unrecorded trials (its streams and its lanes' initial barrier
+    states carry over, as in the reference's trial loop) and the results are
+    the last trial's."""
     nsteps_eff = (p.nsteps // p.save_freq) * p.save_freq if p.save_freq >= 10 else p.nsteps
     cap = cap or (nsteps_eff + 8)
     banks = NumpyBanks(seed)
@@ -36,6 +39,8 @@ def run_rank_recorded(oracle, p, W, seed, bmin=-300, bn=600, cap=None, bank=4096
     fields = 7 if p.save_all else 1
     frames = np.zeros((fields, nfr, W))
     nf = C.c_uint64()
+    for _ in range(trials - 1):
+        oracle.ldo_rank_integrate(st, lanes, ol.dptr(frames), nfr, C.byref(nf), None, None)
     rec_g = np.zeros((W, cap))
     rec_f = np.zeros((W, cap))
     rec_c = np.zeros((W, cap))

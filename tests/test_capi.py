@@ -38,16 +38,18 @@ def test_library_loads_without_gpu():
 def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "sizes.c"
     src.write_text('#include "ldpcp.h"\n#include <stdio.h>\n#include <stddef.h>\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ldpcp_run_params), '
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ldpcp_run_params), '
                    'sizeof(ldpcp_lane), sizeof(ldpcp_sweep_set), sizeof(ldpcp_set_stats), '
                    'offsetof(ldpcp_lane, wt_tick_prev), offsetof(ldpcp_run_params, barrier_window), '
                    'sizeof(ldpcp_engine_config), sizeof(ldpcp_server_config), offsetof(ldpcp_set_stats, hist), '
-                   'offsetof(ldpcp_server_config, engine));return 0;}\n')
+                   'offsetof(ldpcp_server_config, engine), sizeof(ldpcp_checkpoint_header), '
+                   'offsetof(ldpcp_checkpoint_header, timestep));return 0;}\n')
     exe = tmp_path / "sizes"
     subprocess.check_call(["gcc", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)])
     got = [int(v) for v in subprocess.check_output([str(exe)], text=True).split()]
     import ctypes as C
     want = [C.sizeof(N.RunParams), C.sizeof(N.LaneResult), C.sizeof(N.SweepSet), C.sizeof(N.SetStats),
             N.LaneResult.wt_tick_prev.offset, N.RunParams.barrier_window.offset, C.sizeof(N.EngineConfig),
-            C.sizeof(N.ServerConfig), N.SetStats.hist.offset, N.ServerConfig.engine.offset]
+            C.sizeof(N.ServerConfig), N.SetStats.hist.offset, N.ServerConfig.engine.offset,
+            C.sizeof(N.CheckpointHeader), N.CheckpointHeader.timestep.offset]
     assert got == want

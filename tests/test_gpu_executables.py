@@ -266,3 +266,79 @@ def test_double_barrier_executable_matches_oracle(oracle, tmp_path, over):
     msd = np.loadtxt(tmp_path / "dbl_msd.log", comments="#")
     np.testing.assert_allclose(msd[:, 0], 0.2 * lags, rtol=1e-6)
     np.testing.assert_allclose(msd[:, 1], total / len(lanes), rtol=1e-6, atol=6e-8)
+
+
+def _outputs(d):
+    """Every output file of a run, header lines (`# key = value`) and the
+    timing log left out: numbers only."""
+    out = {}
+    for name in sorted(os.listdir(d)):
+        if not name.startswith("out_") or name.endswith("_time.log"):
+            continue
+        if name.endswith(".dat"):
+            out[name] = (d / name).read_bytes()
+        else:
+            out[name] = [ln for ln in (d / name).read_text().splitlines() if not (ln.startswith("#") and "=" in ln)]
+    return out
+
+
+@pytest.mark.parametrize("model", ["single", "double"])
+def test_executable_restart_continues_exactly(tmp_path, model):
+    """WRITERESTART after 100000 steps, then READRESTART for 100000 more
+    (NUMBEROFPREVIOUSSTEPS 100000), gives every output file of one
+    200000-step run: means, logs, MSD over all frames, trajectory logs."""
+    base = (SINGLE_INP.format(over="true") if model == "single" else DOUBLE_INP.format(over="false"))
+    base = re.sub(r"(?m)^OUTPUTBASE .*$", "OUTPUTBASE out", base)
+    base = re.sub(r"(?m)^FREQSAVE .*$", "FREQSAVE 10", base)
+    base = re.sub(r"(?m)^NUMBEROFRUNS .*$", "NUMBEROFRUNS 2", base)
+    exe = os.path.join(BIN, "langevinDynamicsSingleBarrier" if model == "single" else "langevinDynamicsDoubleBarrier")
+
+    def run(d, **keys):
+        text = base
+        for k, v in keys.items():
+            text = re.sub(rf"(?m)^{k} .*$", f"{k} {v}", text)
+        (d / "run.inp").write_text(text)
+        print(f"[gpu exe restart] {model} {d.name} {keys}", flush=True)
+        r = subprocess.run([exe, "-i", "run.inp", "--ranks", "2", "--simd-width", "4"], cwd=d, capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0 and "Error" not in r.stderr, r.stderr[-2000:]
+
+    full, split = tmp_path / "full", tmp_path / "split"
+    full.mkdir()
+    split.mkdir()
+    run(full, NUMBEROFSTEPS=200000)
+    run(split, NUMBEROFSTEPS=100000, WRITERESTART="true", OUTPUTRESTART="rs")
+    restarts = sorted(f for f in os.listdir(split) if f.endswith(".restart"))
+    assert restarts == [f"rs_{r:05d}_{t:05d}.restart" for r in range(2) for t in range(2)]
+    for f in os.listdir(split):
+        if f.startswith("out_"):
+            os.remove(split / f)
+    run(split, NUMBEROFSTEPS=100000, NUMBEROFPREVIOUSSTEPS=100000, READRESTART="true", INPUTRESTART="rs")
+    want, got = _outputs(full), _outputs(split)
+    assert sorted(got) == sorted(want)
+    for name in want:
+        assert got[name] == want[name], name
+
+
+def test_executable_restart_checks_settings(tmp_path):
+    """A restart file written under other settings is refused with the
+    reference's message, and the run ends (exit status 0, as the reference)."""
+    base = re.sub(r"(?m)^FREQSAVE .*$", "FREQSAVE 10", SINGLE_INP.format(over="true"))
+    base = re.sub(r"(?m)^NUMBEROFRUNS .*$", "NUMBEROFRUNS 1", base)
+    base = re.sub(r"(?m)^WRITEX .*$", "WRITEX false", base)
+    exe = os.path.join(BIN, "langevinDynamicsSingleBarrier")
+    first = re.sub(r"(?m)^WRITERESTART .*$", "WRITERESTART true", base).replace("OUTPUTRESTART        none",
+                                                                             "OUTPUTRESTART        rs")
+    first = re.sub(r"(?m)^NUMBEROFSTEPS .*$", "NUMBEROFSTEPS 100000", first)
+    (tmp_path / "a.inp").write_text(first)
+    r = subprocess.run([exe, "-i", "a.inp", "--ranks", "1", "--simd-width", "4"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0 and (tmp_path / "rs_00000_00000.restart").exists(), r.stderr[-2000:]
+    second = re.sub(r"(?m)^READRESTART .*$", "READRESTART true", base).replace("INPUTRESTART         none",
+                                                                           "INPUTRESTART         rs")
+    second = re.sub(r"(?m)^TIMESTEP .*$", "TIMESTEP 0.002", second)
+    (tmp_path / "b.inp").write_text(second)
+    r = subprocess.run([exe, "-i", "b.inp", "--ranks", "1", "--simd-width", "4"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0
+    assert "Error: inconsistent time step0" in r.stderr and "Error: can't read restart file on rank" in r.stderr

@@ -1,0 +1,163 @@
+"""Restart on the GPU: checkpoint images and resumed runs (run mode).
+
+The reference writes its whole trajectory state at the end of a run
+(WriteRestartFile, langevinDynamicsSingleBarrier.h:1048-1136) and a later run
+with READRESTART continues it (ReadRestartFile :1137-1206; the resumed
+Integrate skips step 0 and continues the step count, :1397-1400, 1420).  Here
+the state is the library's checkpoint image and the noise position is the step
+count itself, so:
+
+* a run of a steps, a checkpoint and a resumed run of b steps reproduce one
+  run of a + b steps bit for bit: every frame of every field, every per-lane
+  statistic, the barrier ticks and the MSD over all frames;
+* in replay mode the resumed run also reproduces the oracle's rank, so the
+  continuation is pinned to the reference's semantics, not only to itself;
+* UNIFORMINITALSTATES barriers created in a later trial take the previous
+  trial's initial state (UserData::m_bInitialValue, :1378-1380).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from langevindynamicsforpcp_amd import LdpcpError
+from langevindynamicsforpcp_amd import _native as N
+from streams import run_rank_recorded
+from test_gpu_run import COUNTS, EXACT, to_engine
+from test_oracle_modes import make
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["single_over", "single_under", "single_eq_false", "single_tau0", "double_over", "double_under"]
+
+
+def lane_dict(lane):
+    out = {}
+    for name, _ in N.LaneResult._fields_:
+        v = getattr(lane, name)
+        out[name] = list(v) if hasattr(v, "__len__") else v
+    return out
+
+
+def engine(name, nsteps, n, seed, spl):
+    le = to_engine(make(name, nsteps=nsteps, save_freq=100, save_all=1), n, seed=seed)
+    le.tick_capacity = 4096
+    le.steps_per_launch = spl
+    return le
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_resume_continues_bit_exact(name):
+    n, seed, a, b = 48, 21, 30000, 45000
+    print(f"[gpu restart] {name} n={n} a={a} b={b}", flush=True)
+    full = engine(name, a + b, n, seed, 6007)
+    full.Integrate(0)
+    first = engine(name, a, n, seed, 7919)
+    first.Integrate(0)
+    image = first.Checkpoint()
+    hd = N.CheckpointHeader.from_buffer_copy(image)
+    assert hd.magic == N.CHECKPOINT_MAGIC and hd.steps_done == a and hd.n_frames == a // 100 + 1
+    assert hd.bytes == len(image)
+    second = engine(name, b, n, seed, 5003)
+    second.Resume(image)
+    second.Integrate(0)
+
+    for t, (x, y) in enumerate(zip(full.lanes(), second.lanes())):
+        assert lane_dict(x) == lane_dict(y), (name, t)
+    assert sum(full.lanes()[t].fpt2_n + full.lanes()[t].cg_n[0] for t in range(n)) > 0
+    for field in range(7):
+        np.testing.assert_array_equal(second.frames(field), full.frames(field), err_msg=f"field {field}")
+    for x, y in zip(full.ticks(), second.ticks()):
+        np.testing.assert_array_equal(x, y)
+    lags = np.array([0, 1, 2, 5, 10, 50, 200, 600], np.uint64)
+    np.testing.assert_array_equal(second.msd(lags), full.msd(lags))
+    for le in (full, first, second):
+        le.close()
+
+
+def test_gpu_checkpoint_of_a_resumed_handle_is_the_image():
+    n, seed = 16, 4
+    first = engine("double_under", 20000, n, seed, 4096)
+    first.Integrate(0)
+    image = first.Checkpoint()
+    second = engine("double_under", 10000, n, seed, 4096)
+    second.Resume(image)
+    second._create()                         # resumed, not yet integrated
+    assert second.Checkpoint() == image
+    second.Integrate(0)
+    with pytest.raises(LdpcpError, match="already continued"):
+        N.check(N.load().ldpcp_run_integrate(second._h, 4096), "ldpcp_run_integrate")
+    first.close()
+    second.close()
+
+
+def test_gpu_resume_rejects_a_different_run():
+    first = engine("single_over", 10000, 16, 5, 4096)
+    first.Integrate(0)
+    image = first.Checkpoint()
+    other = engine("single_over", 10000, 16, 6, 4096)   # another seed: another run
+    other.Resume(image)
+    with pytest.raises(LdpcpError, match="inconsistent seed"):
+        other.Integrate(0)
+    fewer = engine("single_over", 10000, 8, 5, 4096)
+    fewer.Resume(image)
+    with pytest.raises(LdpcpError, match="inconsistent trajectory count"):
+        fewer.Integrate(0)
+    bad = bytearray(image)
+    bad[0] ^= 1
+    third = engine("single_over", 10000, 16, 5, 4096)
+    third.Resume(bytes(bad))
+    with pytest.raises(LdpcpError, match="not a checkpoint image"):
+        third.Integrate(0)
+    for le in (first, other, fewer, third):
+        le.close()
+
+
+@pytest.mark.parametrize("name", ["single_over", "double_under"])
+def test_gpu_resumed_replay_matches_oracle(oracle, name):
+    """Replay of a reference-style rank, split in two runs at a checkpoint."""
+    W, a, b = 8, 20000, 40000
+    p = make(name, nsteps=a + b, save_freq=100, save_all=1)
+    print(f"[gpu restart replay] {name} W={W}", flush=True)
+    lanes, frames, s = run_rank_recorded(oracle, p, W, seed=13)
+    ng, nf = min(s["n_gauss"]), min(s["n_flip"])
+    replay = (s["gauss"][:, :ng].T, s["flip"][:, :nf].T, s["catch"], s["binit"], s["bmin"])
+    first = to_engine(make(name, nsteps=a, save_freq=100, save_all=1), W)
+    first.barrier_window = 1024
+    first.SetReplay(*replay)
+    first.Integrate(0)
+    second = to_engine(make(name, nsteps=b, save_freq=100, save_all=1), W)
+    second.barrier_window = 1024
+    second.SetReplay(*replay)
+    second.Resume(first.Checkpoint())
+    second.Integrate(0)
+    got = second.lanes()
+    for field in range(7):
+        np.testing.assert_array_equal(second.frames(field).T, frames[field], err_msg=f"frame field {field}")
+    for i in range(W):
+        for f in EXACT + COUNTS:
+            assert getattr(got[i], f) == getattr(lanes[i], f), (name, i, f)
+        assert list(got[i].cg_sum) == list(lanes[i].cg_sum)
+    first.close()
+    second.close()
+
+
+def test_gpu_same_initial_barrier_second_trial_matches_oracle(oracle):
+    """Trial 2 of a rank with UNIFORMINITALSTATES: barriers created during the
+    trial take trial 1's initial states (1 here), not 0."""
+    W = 8
+    p = make("single_same_init", nsteps=60000, save_freq=100, save_all=1)
+    lanes, frames, s = run_rank_recorded(oracle, p, W, seed=17, trials=2)
+    ng, nf = min(s["n_gauss"]), min(s["n_flip"])
+    le = to_engine(p, W)
+    le.barrier_window = 1024
+    le.SetReplay(s["gauss"][:, :ng].T, s["flip"][:, :nf].T, s["catch"], s["binit"], s["bmin"])
+    le.SetBarrierDefault(np.full(W, p.initial_state))
+    le.Integrate(0)
+    got = le.lanes()
+    assert sum(lanes[i].n_catch for i in range(W)) > 0
+    np.testing.assert_array_equal(le.frames(0).T, frames[0])
+    for i in range(W):
+        for f in EXACT + COUNTS:
+            assert getattr(got[i], f) == getattr(lanes[i], f), (i, f)
+    le.close()
