@@ -58,6 +58,14 @@ struct RunPtrs {
     double* bdefault;    // state of barriers created with UNIFORMINITALSTATES [traj]
 };
 
+// The run's branch flags, separate from RunConst so that a kernel can make them
+// compile-time constants while the constants stay in the kernel arguments.
+struct Flags {
+    int model, over, tau0, replay;
+};
+
+__device__ __forceinline__ Flags run_flags(const RunConst& c) { return {c.model, c.overdamped, c.tau_zero, c.replay}; }
+
 // Constants of IntegrateInitializeVariables (langevinDynamicsSingleBarrier.h:1520-1618,
 // overdamped :2812-2824; langevinDynamicsDoubleBarrier.h:1733-1937, GetXc :1722-1732).
 void make_const(const ldpcp_run_params& p, RunConst& c)
@@ -169,10 +177,10 @@ struct Lane {
 };
 
 // UpdateBarrierStateEtaParamter (Single.h:2154-2165; Double.h:2737-2757)
-__device__ __forceinline__ void lane_param(Lane& s, const RunConst& c, bool count = true)
+__device__ __forceinline__ void lane_param(Lane& s, const RunConst& c, const Flags& f, bool count = true)
 {
     const bool one = (s.state == 1.0);
-    if (c.model == 0) {
+    if (f.model == 0) {
         s.height = one ? c.H[1] : c.H[0];
         s.slope_pos = one ? c.S[1] : c.S[0];
     } else {
@@ -193,9 +201,9 @@ __device__ __forceinline__ void lane_param(Lane& s, const RunConst& c, bool coun
 }
 
 // EnergyAndForce (Single.h:1292-1324; Double.h:1464-1522)
-__device__ __forceinline__ void lane_energy_force(Lane& s, const RunConst& c)
+__device__ __forceinline__ void lane_energy_force(Lane& s, const RunConst& c, const Flags& f)
 {
-    if (c.model == 0) {
+    if (f.model == 0) {
         const double r = s.x * c.Linv;
         const double pidx = rint(r);
         const double xp = s.x - pidx * c.L;
@@ -241,7 +249,7 @@ __device__ __forceinline__ void lane_energy_force(Lane& s, const RunConst& c)
 }
 
 // InitializePeriodicIndices / UpdatePeriodicIndices (Single.h:1838-1884; Double.h:2150-2235)
-__device__ __forceinline__ void lane_indices(Lane& s, const RunConst& c, bool init)
+__device__ __forceinline__ void lane_indices(Lane& s, const RunConst& c, const Flags& f, bool init)
 {
     const double r = s.x * c.Linv;
     const double well = floor(r), periodic = rint(r);
@@ -257,7 +265,7 @@ __device__ __forceinline__ void lane_indices(Lane& s, const RunConst& c, bool in
         s.ent_well = (s.well_prev != s.well);
         s.ent_period = (s.periodic_prev != s.periodic);
     }
-    if (c.model == 1) {
+    if (f.model == 1) {
         s.xp = s.x - s.periodic * c.L;
         const double local = (s.xp < c.xc) ? 0.0 : 1.0;
         if (init) {
@@ -282,13 +290,20 @@ struct StepNoise {
     uint32_t w3;
 };
 
-__device__ __forceinline__ void step_noise(const RunConst& c, const RunPtrs& P, uint32_t t, uint64_t iStep,
-                                           StepNoise& sn, double& g, double& u)
+// replayed noise of step iStep (>= 1)
+__device__ __forceinline__ void replay_noise(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
+                                             uint64_t iStep, double& g, double& u)
 {
-    if (c.replay) {
-        const uint64_t k = iStep - 1;
-        g = P.g[(c.init_gauss + k) * c.n_traj + t];
-        u = c.tau_zero ? 1.0 : P.f[(c.init_flip + k) * c.n_traj + t];
+    const uint64_t k = iStep - 1;
+    g = P.g[(c.init_gauss + k) * c.n_traj + t];
+    u = f.tau0 ? 1.0 : P.f[(c.init_flip + k) * c.n_traj + t];
+}
+
+__device__ __forceinline__ void step_noise(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
+                                           uint64_t iStep, StepNoise& sn, double& g, double& u)
+{
+    if (f.replay) {
+        replay_noise(c, f, P, t, iStep, g, u);
         return;
     }
     const uint64_t q = (iStep - 1) >> 1;
@@ -310,14 +325,14 @@ __device__ __forceinline__ void step_noise(const RunConst& c, const RunPtrs& P, 
     }
 }
 
-__device__ __forceinline__ int barrier_init_state(const RunConst& c, const RunPtrs& P, uint32_t t, int array, int id,
-                                                  int& err)
+__device__ __forceinline__ int barrier_init_state(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
+                                                  int array, int id, int& err)
 {
     // UserData::m_bInitialValue: the lane's BarrierState0Instance as it stood before step 0
     // (Single.h:1378-1380, 80-84): 0 for a fresh integrator, the previous trial's or the
     // restart file's initial state otherwise
     if (c.same_init) return (int)P.bdefault[t];
-    if (c.replay) {
+    if (f.replay) {
         const int k = id - c.bmin;
         if (k < 0 || k >= c.bn) {
             err = 2;
@@ -346,12 +361,12 @@ __device__ __forceinline__ uint64_t* win_entry(const RunConst& c, const RunPtrs&
     return P.win + ((uint64_t)t * 2 + array) * c.wb + k;
 }
 
-__device__ __forceinline__ uint64_t win_load(const RunConst& c, const RunPtrs& P, uint32_t t, int array, int id,
-                                             uint64_t* e, int& err)
+__device__ __forceinline__ uint64_t win_load(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
+                                             int array, int id, uint64_t* e, int& err)
 {
     uint64_t v = *e;
     if (!(v & kValid)) {
-        v = kValid | (barrier_init_state(c, P, t, array, id, err) ? kState : 0ull);  // created at step 0
+        v = kValid | (barrier_init_state(c, f, P, t, array, id, err) ? kState : 0ull);  // created at step 0
         *e = v;
     }
     return v;
@@ -372,7 +387,7 @@ __device__ __forceinline__ void arm_indices(int idx[2], const Lane& s)
     }
 }
 
-__device__ void stats_init(const RunConst& c, ldpcp_lane* o, const Lane& s)
+__device__ __forceinline__ void stats_init(const RunConst& c, ldpcp_lane* o, const Lane& s)
 {
     if (c.b_wait) {
         o->wt_tick_prev = 0.0;
@@ -417,10 +432,11 @@ __device__ __forceinline__ double ev(const RunConst& c, uint64_t iStep, double l
     return fma(c.dt, (double)iStep, -last);
 }
 
-__device__ void stats_event(const RunConst& c, ldpcp_lane* o, const Lane& s, uint64_t iStep, double* ticks)
+__device__ __forceinline__ void stats_event(const RunConst& c, const Flags& f, ldpcp_lane* o, const Lane& s,
+                                            uint64_t iStep, double* ticks)
 {
     const double t = c.dt * (double)iStep;
-    const bool single = (c.model == 0);
+    const bool single = (f.model == 0);
     if (c.b_wait && single) {
         if (s.ent_well) {
             o->wt_tick_prev = (double)iStep;
@@ -510,14 +526,15 @@ __device__ void stats_event(const RunConst& c, ldpcp_lane* o, const Lane& s, uin
 }
 
 // Record (Single.h:1707-1765); frames are [field][frame][traj]
-__device__ __forceinline__ void record(const RunConst& c, const RunPtrs& P, uint32_t t, const Lane& s, uint64_t frame)
+__device__ __forceinline__ void record(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t, const Lane& s,
+                                       uint64_t frame)
 {
     if (frame >= c.frames_cap) return;
     const uint64_t plane = c.frames_cap * c.n_traj;
     const uint64_t o = frame * c.n_traj + t;
     P.frames[o] = s.x;
     if (c.save_all) {
-        P.frames[plane * 1 + o] = c.overdamped ? 0.0 : s.v;
+        P.frames[plane * 1 + o] = f.over ? 0.0 : s.v;
         P.frames[plane * 2 + o] = s.F;
         P.frames[plane * 3 + o] = s.E;
         P.frames[plane * 4 + o] = c.mhalf * (s.v * s.v);
@@ -526,7 +543,7 @@ __device__ __forceinline__ void record(const RunConst& c, const RunPtrs& P, uint
     }
 }
 
-__device__ __forceinline__ void load_lane(Lane& s, const RunConst& c, const RunPtrs& P, uint32_t t)
+__device__ __forceinline__ void load_lane(Lane& s, const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t)
 {
     s.x = P.x[t];
     s.v = P.v[t];
@@ -542,7 +559,7 @@ __device__ __forceinline__ void load_lane(Lane& s, const RunConst& c, const RunP
     s.cnt0 = P.cnt0[t];
     s.cnt1 = P.cnt1[t];
     s.ent_well = s.ent_period = s.ent_local = false;
-    lane_param(s, c, false);  // rebuild heights and slopes from the stored state
+    lane_param(s, c, f, false);  // rebuild heights and slopes from the stored state
 }
 
 __device__ __forceinline__ void store_lane(const Lane& s, const RunPtrs& P, uint32_t t)
@@ -565,6 +582,7 @@ __device__ __forceinline__ void store_lane(const Lane& s, const RunPtrs& P, uint
 // IntegrateStep0 (Single.h:1619-1652, overdamped :2825-2832)
 __global__ void __launch_bounds__(256) run_init_kernel(RunConst c, RunPtrs P)
 {
+    const Flags f = run_flags(c);
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= c.n_traj) return;
     ldpcp_lane* o = P.lanes + t;
@@ -573,12 +591,12 @@ __global__ void __launch_bounds__(256) run_init_kernel(RunConst c, RunPtrs P)
     s.v = s.F = s.E = s.R = s.barrier = 0.0;
     s.xp = s.local = s.local_prev = 0.0;
     s.x = c.x0;
-    lane_indices(s, c, true);
+    lane_indices(s, c, f, true);
     Philox4 w0;
-    if (!c.replay) w0 = philox4x32_10(0u, 0u, c.traj_offset + t, 1u, c.key0, c.key1);
+    if (!f.replay) w0 = philox4x32_10(0u, 0u, c.traj_offset + t, 1u, c.key0, c.key1);
     uint64_t ng = 0, nf = 0;
     if (c.randomize_state) {
-        const double u = c.replay ? P.f[t] : unit_f64(w0.w[2]);
+        const double u = f.replay ? P.f[t] : unit_f64(w0.w[2]);
         nf++;
         s.state = (u <= c.p0) ? 0.0 : 1.0;
     } else {
@@ -587,30 +605,30 @@ __global__ void __launch_bounds__(256) run_init_kernel(RunConst c, RunPtrs P)
     o->state0_instance = s.state;
     P.wbase[t] = (int)s.periodic - c.wb / 2;
     {
-        const int a = c.model == 0 ? 0 : (int)s.local;
+        const int a = f.model == 0 ? 0 : (int)s.local;
         uint64_t* e = win_entry(c, P, t, a, (int)s.periodic, err);
         if (e) *e = kValid | ((int)s.state ? kState : 0ull);
     }
     s.cnt0 = s.cnt1 = 0;
-    lane_param(s, c);
+    lane_param(s, c, f);
     double gv = 0.0, gr = 0.0;
-    if (!c.replay) box_muller_f64(w0.w[0], w0.w[1], gv, gr);
+    if (!f.replay) box_muller_f64(w0.w[0], w0.w[1], gv, gr);
     if (c.randomize_v) {
-        const double g = c.replay ? P.g[ng * c.n_traj + t] : gv;
+        const double g = f.replay ? P.g[ng * c.n_traj + t] : gv;
         ng++;
         s.v = c.vsigma * g;
     } else {
         s.v = c.v0;
     }
     o->v0_instance = s.v;
-    lane_energy_force(s, c);
+    lane_energy_force(s, c, f);
     stats_init(c, o, s);
-    if (c.overdamped) {
-        const double g = c.replay ? P.g[ng * c.n_traj + t] : gr;
+    if (f.over) {
+        const double g = f.replay ? P.g[ng * c.n_traj + t] : gr;
         ng++;
         s.R = c.var * g;
     }
-    record(c, P, t, s, 0);
+    record(c, f, P, t, s, 0);
     o->n_gauss = ng;
     o->n_flip = nf;
     o->error = err;
@@ -618,16 +636,22 @@ __global__ void __launch_bounds__(256) run_init_kernel(RunConst c, RunPtrs P)
 }
 
 // Steps (s0, s1] of IntegrateStepEta / IntegrateStep
-// (underdamped Single.h:1654-1706; overdamped :2833-2886; Double.h:1971-2024, 3475-3508)
+// (underdamped Single.h:1654-1706; overdamped :2833-2886; Double.h:1971-2024, 3475-3508).
+// One instantiation per (model, integrator, TAU = 0, noise source): the flags
+// become constants, so the step is straight-line code without the branches and
+// constant reloads of a generic loop (a handle with few trajectories runs one
+// wavefront, and its time per step is the latency of that code).
+template <int MODEL, int OVER, int TAU0, int REPLAY>
 __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, uint64_t s0, uint64_t s1)
 {
+    constexpr Flags f{MODEL, OVER, TAU0, REPLAY};
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= c.n_traj) return;
     ldpcp_lane* o = P.lanes + t;
     if (o->error) return;
     int err = 0;
     Lane s;
-    load_lane(s, c, P, t);
+    load_lane(s, c, f, P, t);
     StepNoise sn;
     sn.q = 0;
     sn.g1 = 0.0;
@@ -636,11 +660,31 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     // frames every save_freq steps: a countdown instead of a 64-bit modulo per step
     uint64_t to_save = c.save_freq - s0 % c.save_freq;
     uint64_t frame = s0 / c.save_freq + 1;
+    // replay: the noise of the next kAhead steps is in flight while a step runs
+    // (indices clamp to s1, which the launch keeps above s0)
+    constexpr int kAhead = 4;
+    double gq[kAhead], uq[kAhead];
+    if constexpr (REPLAY) {
+#pragma unroll
+        for (int d = 0; d < kAhead; ++d)
+            replay_noise(c, f, P, t, (s0 + 1 + d < s1) ? s0 + 1 + d : s1, gq[d], uq[d]);
+    }
     uint64_t iStep;
     for (iStep = s0 + 1; iStep <= s1; ++iStep) {
         double g, u;
-        step_noise(c, P, t, iStep, sn, g, u);
-        if (c.overdamped) {
+        if constexpr (REPLAY) {
+            g = gq[0];
+            u = uq[0];
+#pragma unroll
+            for (int d = 0; d + 1 < kAhead; ++d) {
+                gq[d] = gq[d + 1];
+                uq[d] = uq[d + 1];
+            }
+            replay_noise(c, f, P, t, (iStep + kAhead < s1) ? iStep + kAhead : s1, gq[kAhead - 1], uq[kAhead - 1]);
+        } else {
+            step_noise(c, f, P, t, iStep, sn, g, u);
+        }
+        if (f.over) {
             const double rf = c.var * g;
             s.x = fma(s.F, c.dtgm, s.x);
             s.x = s.x + rf;
@@ -652,9 +696,9 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
             s.v = c.expmgdt * s.v + c.var * g;
             s.x = s.x + s.v * c.halfdt;
         }
-        lane_indices(s, c, false);
+        lane_indices(s, c, f, false);
         // per-step flip of the barrier under the particle (Single.h:2010-2015, 2078-2086)
-        if (!c.tau_zero) {
+        if (!f.tau0) {
             if (c.equal_prob) {
                 if (u < c.p_flip) s.state = (s.state == 0.0) ? 1.0 : 0.0;
             } else {
@@ -663,17 +707,17 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
             }
         }
         // hand-over to the barrier just entered (Single.h:2025-2068; Double.h:2466-2510)
-        const bool entering = c.model == 0 ? s.ent_period : s.ent_local;
+        const bool entering = f.model == 0 ? s.ent_period : s.ent_local;
         if (entering) {
-            const int ap = c.model == 0 ? 0 : (int)s.local_prev;
-            const int ac = c.model == 0 ? 0 : (int)s.local;
+            const int ap = f.model == 0 ? 0 : (int)s.local_prev;
+            const int ac = f.model == 0 ? 0 : (int)s.local;
             uint64_t* ep = win_entry(c, P, t, ap, (int)s.periodic_prev, err);
             uint64_t* ec = win_entry(c, P, t, ac, (int)s.periodic, err);
             if (err) break;
             *ep = kValid | ((int)s.state ? kState : 0ull) | (iStep & kStepMask);
-            const uint64_t ev = win_load(c, P, t, ac, (int)s.periodic, ec, err);
+            const uint64_t ev = win_load(c, f, P, t, ac, (int)s.periodic, ec, err);
             const double prev_state = (ev & kState) ? 1.0 : 0.0;
-            if (c.tau_zero) {
+            if (f.tau0) {
                 s.state = prev_state;
             } else {
                 const double lap = (double)(iStep - (ev & kStepMask));
@@ -682,7 +726,7 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
                 else if (prev_state == 0.0) pc = c.K / (1.0 + c.K) * (1.0 - exp(c.c_lap * lap));
                 else pc = 1.0 / (1.0 + c.K) * (1.0 - exp(c.c_lap * lap));
                 double uc;
-                if (c.replay) {
+                if (f.replay) {
                     if (n_catch >= c.catch_cap) {
                         err = 4;
                         break;
@@ -698,18 +742,18 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
             }
             if (err) break;
         }
-        lane_param(s, c);
-        lane_energy_force(s, c);
-        if (!c.overdamped) s.v = s.v + (c.minv * s.F) * c.halfdt;
-        if (s.ent_period || s.ent_well) stats_event(c, o, s, iStep, P.ticks + (uint64_t)t * c.tick_cap);
+        lane_param(s, c, f);
+        lane_energy_force(s, c, f);
+        if (!f.over) s.v = s.v + (c.minv * s.F) * c.halfdt;
+        if (s.ent_period || s.ent_well) stats_event(c, f, o, s, iStep, P.ticks + (uint64_t)t * c.tick_cap);
         if (--to_save == 0) {
-            record(c, P, t, s, frame++);
+            record(c, f, P, t, s, frame++);
             to_save = c.save_freq;
         }
     }
     o->n_catch = n_catch;
     o->n_gauss += (iStep - 1) - s0;
-    if (!c.tau_zero) o->n_flip += (iStep - 1) - s0;
+    if (!f.tau0) o->n_flip += (iStep - 1) - s0;
     o->steps_done = iStep - 1;
     o->error = err;
     store_lane(s, P, t);
@@ -734,6 +778,21 @@ __global__ void __launch_bounds__(256) msd_kernel(const double* __restrict__ fra
         acc /= (double)(n - i);
     }
     out[(uint64_t)t * nlags + l] = acc;
+}
+
+using StepKernel = void (*)(RunConst, RunPtrs, uint64_t, uint64_t);
+
+template <int MODEL, int OVER>
+StepKernel step_kernel_noise(const RunConst& c)
+{
+    if (c.tau_zero) return c.replay ? run_step_kernel<MODEL, OVER, 1, 1> : run_step_kernel<MODEL, OVER, 1, 0>;
+    return c.replay ? run_step_kernel<MODEL, OVER, 0, 1> : run_step_kernel<MODEL, OVER, 0, 0>;
+}
+
+StepKernel step_kernel_for(const RunConst& c)
+{
+    if (c.model == 0) return c.overdamped ? step_kernel_noise<0, 1>(c) : step_kernel_noise<0, 0>(c);
+    return c.overdamped ? step_kernel_noise<1, 1>(c) : step_kernel_noise<1, 0>(c);
 }
 
 }  // namespace
@@ -938,7 +997,7 @@ extern "C" int ldpcp_run_integrate(ldpcp_run* h, uint64_t steps_per_launch)
     const uint64_t end = h->step_base + h->c.nsteps_eff;
     for (uint64_t s0 = h->step_base; s0 < end; s0 += steps_per_launch) {
         const uint64_t s1 = (s0 + steps_per_launch < end) ? s0 + steps_per_launch : end;
-        hipLaunchKernelGGL(run_step_kernel, grid, block, 0, h->stream, h->c, h->P, s0, s1);
+        step_kernel_for(h->c)<<<grid, block, 0, h->stream>>>(h->c, h->P, s0, s1);
         LDPCP_HIP_TRY(hipGetLastError());
     }
     LDPCP_HIP_TRY(hipStreamSynchronize(h->stream));
