@@ -641,9 +641,32 @@ __global__ void __launch_bounds__(256) run_init_kernel(RunConst c, RunPtrs P)
 // become constants, so the step is straight-line code without the branches and
 // constant reloads of a generic loop (a handle with few trajectories runs one
 // wavefront, and its time per step is the latency of that code).
-template <int MODEL, int OVER, int TAU0, int REPLAY>
-__global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, uint64_t s0, uint64_t s1)
+//
+// Noise sources: kNoisePhilox draws Philox4x32-10 + Box-Muller inside the step
+// loop; kNoiseReplay reads recorded streams; kNoiseBuffered reads the same
+// Philox noise that run_noise_kernel wrote for the launch's steps, staged
+// through LDS 32 steps at a time by direct global->LDS loads, so that the
+// stepping wavefront's dependent chain holds the dynamics only (the noise is
+// computed in parallel over all trajectory-steps beforehand).
+constexpr int kNoisePhilox = 0, kNoiseReplay = 1, kNoiseBuffered = 2;
+constexpr int kStage = 32;  // steps per LDS stage of the buffered noise
+// s_waitcnt immediate (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4,
+// lgkmcnt 11:8) that waits for vmcnt = 0 and leaves the other counters free
+constexpr int kWaitVmcnt0 = 0x0F70;
+
+// One buffered noise record per trajectory-step: Gaussian (lo, hi words) and
+// the 32-bit flip word.
+__device__ __forceinline__ void buffered_noise(const uint4& r, double& g, double& u)
 {
+    g = __hiloint2double((int)r.y, (int)r.x);
+    u = unit_f64(r.z);
+}
+
+template <int MODEL, int OVER, int TAU0, int SRC>
+__global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, uint64_t s0, uint64_t s1,
+                                                       const uint4* __restrict__ noise)
+{
+    constexpr int REPLAY = (SRC == kNoiseReplay);
     constexpr Flags f{MODEL, OVER, TAU0, REPLAY};
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= c.n_traj) return;
@@ -669,10 +692,41 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         for (int d = 0; d < kAhead; ++d)
             replay_noise(c, f, P, t, (s0 + 1 + d < s1) ? s0 + 1 + d : s1, gq[d], uq[d]);
     }
+    // buffered noise: stage j of the launch holds steps s0 + 1 + 32 j ...; the
+    // launch is one wavefront per workgroup (64 threads), so a stage's
+    // __syncthreads only waits for this wavefront's own loads
+    [[maybe_unused]] const uint32_t lane = threadIdx.x;
+    typedef uint4 StageBuf[2][kStage][64];
+    [[maybe_unused]] StageBuf* stage = nullptr;
+    if constexpr (SRC == kNoiseBuffered) {
+        __shared__ uint4 stage_mem[2][kStage][64];
+        stage = &stage_mem;
+        const uint64_t len = s1 - s0;
+        const uint64_t m = len < kStage ? len : kStage;
+        for (uint64_t j = 0; j < m; ++j)
+            __builtin_amdgcn_global_load_lds((const void*)(noise + j * c.n_traj + t),
+                                             (__attribute__((address_space(3))) void*)&stage_mem[0][j][0], 16, 0, 0);
+    }
     uint64_t iStep;
     for (iStep = s0 + 1; iStep <= s1; ++iStep) {
         double g, u;
-        if constexpr (REPLAY) {
+        if constexpr (SRC == kNoiseBuffered) {
+            const uint64_t k = iStep - s0 - 1;  // step within the launch
+            const uint32_t j = (uint32_t)(k % kStage), b = (uint32_t)((k / kStage) & 1);
+            if (j == 0) {
+                // this stage's direct-to-LDS loads have landed: they count in vmcnt
+                // only, which a workgroup barrier does not wait for
+                __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+                __syncthreads();
+                const uint64_t next = k + kStage, len = s1 - s0;
+                const uint64_t m = next >= len ? 0 : (len - next < kStage ? len - next : kStage);
+                for (uint64_t jj = 0; jj < m; ++jj)
+                    __builtin_amdgcn_global_load_lds((const void*)(noise + (next + jj) * c.n_traj + t),
+                                                     (__attribute__((address_space(3))) void*)&(*stage)[b ^ 1][jj][0],
+                                                     16, 0, 0);
+            }
+            buffered_noise((*stage)[b][j][lane], g, u);
+        } else if constexpr (REPLAY) {
             g = gq[0];
             u = uq[0];
 #pragma unroll
@@ -780,20 +834,52 @@ __global__ void __launch_bounds__(256) msd_kernel(const double* __restrict__ fra
     out[(uint64_t)t * nlags + l] = acc;
 }
 
-using StepKernel = void (*)(RunConst, RunPtrs, uint64_t, uint64_t);
+// The Philox noise of steps (s0, s0 + len] for every trajectory, laid out
+// [step][trajectory] as buffered_noise reads it: the values step_noise
+// would draw in the loop (Gaussian of the step's Box-Muller pair half, flip
+// word w[2] / w[3]).
+__global__ void __launch_bounds__(256) run_noise_kernel(RunConst c, uint64_t s0, uint4* __restrict__ noise)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.n_traj) return;
+    const uint64_t k = blockIdx.y;
+    const uint64_t iStep = s0 + 1 + k;
+    const uint64_t q = (iStep - 1) >> 1;
+    const Philox4 w = philox4x32_10((uint32_t)q, (uint32_t)(q >> 32), c.traj_offset + t, 0u, c.key0, c.key1);
+    double g0, g1;
+    box_muller_f64(w.w[0], w.w[1], g0, g1);
+    const bool second = (iStep - 1) & 1;
+    const double g = second ? g1 : g0;
+    const long long bits = __double_as_longlong(g);
+    noise[k * c.n_traj + t] = make_uint4((uint32_t)bits, (uint32_t)(bits >> 32), second ? w.w[3] : w.w[2], 0u);
+}
+
+using StepKernel = void (*)(RunConst, RunPtrs, uint64_t, uint64_t, const uint4*);
+
+template <int MODEL, int OVER, int SRC>
+StepKernel step_kernel_tau(const RunConst& c)
+{
+    return c.tau_zero ? run_step_kernel<MODEL, OVER, 1, SRC> : run_step_kernel<MODEL, OVER, 0, SRC>;
+}
 
 template <int MODEL, int OVER>
-StepKernel step_kernel_noise(const RunConst& c)
+StepKernel step_kernel_noise(const RunConst& c, int src)
 {
-    if (c.tau_zero) return c.replay ? run_step_kernel<MODEL, OVER, 1, 1> : run_step_kernel<MODEL, OVER, 1, 0>;
-    return c.replay ? run_step_kernel<MODEL, OVER, 0, 1> : run_step_kernel<MODEL, OVER, 0, 0>;
+    if (src == kNoiseReplay) return step_kernel_tau<MODEL, OVER, kNoiseReplay>(c);
+    if (src == kNoiseBuffered) return step_kernel_tau<MODEL, OVER, kNoiseBuffered>(c);
+    return step_kernel_tau<MODEL, OVER, kNoisePhilox>(c);
 }
 
-StepKernel step_kernel_for(const RunConst& c)
+StepKernel step_kernel_for(const RunConst& c, int src)
 {
-    if (c.model == 0) return c.overdamped ? step_kernel_noise<0, 1>(c) : step_kernel_noise<0, 0>(c);
-    return c.overdamped ? step_kernel_noise<1, 1>(c) : step_kernel_noise<1, 0>(c);
+    if (c.model == 0) return c.overdamped ? step_kernel_noise<0, 1>(c, src) : step_kernel_noise<0, 0>(c, src);
+    return c.overdamped ? step_kernel_noise<1, 1>(c, src) : step_kernel_noise<1, 0>(c, src);
 }
+
+// Handles with at most this many trajectories fill a small part of the GPU, so
+// each step's latency decides their time: they take the buffered noise.
+constexpr uint32_t kBufferedMaxTraj = 8192;
+constexpr uint64_t kBufferedLaunch = 8192;  // steps per launch (and per noise buffer)
 
 }  // namespace
 
@@ -806,6 +892,7 @@ struct ldpcp_run {
     hipStream_t stream;
     void* state_block;
     void* frames_block;   // frames of a resumed run (prior frames + this run's), else null
+    uint4* noise;         // buffered noise, [kBufferedLaunch][n_traj], allocated on first use
     uint64_t n_frames;
     uint64_t step_base;   // absolute step the next integrate starts after
     int phase;            // 0 fresh, 1 resumed (next integrate continues), 2 resumed run done
@@ -995,9 +1082,22 @@ extern "C" int ldpcp_run_integrate(ldpcp_run* h, uint64_t steps_per_launch)
     // steps (step_base, step_base + NSTEPS]: a resumed run continues the absolute step
     // count (Single.h:1420, 1471), which is also the Philox counter of its noise
     const uint64_t end = h->step_base + h->c.nsteps_eff;
+    const bool buffered = !h->c.replay && n <= kBufferedMaxTraj;
+    const int src = h->c.replay ? kNoiseReplay : buffered ? kNoiseBuffered : kNoisePhilox;
+    const StepKernel step = step_kernel_for(h->c, src);
+    if (buffered) {
+        if (steps_per_launch > kBufferedLaunch) steps_per_launch = kBufferedLaunch;
+        if (!h->noise) LDPCP_HIP_TRY(hipMalloc(&h->noise, kBufferedLaunch * n * sizeof(uint4)));
+    }
+    const dim3 sblock(buffered ? 64 : 256), sgrid((unsigned)((n + sblock.x - 1) / sblock.x));
     for (uint64_t s0 = h->step_base; s0 < end; s0 += steps_per_launch) {
         const uint64_t s1 = (s0 + steps_per_launch < end) ? s0 + steps_per_launch : end;
-        step_kernel_for(h->c)<<<grid, block, 0, h->stream>>>(h->c, h->P, s0, s1);
+        if (buffered) {
+            const dim3 ngrid((unsigned)((n + 255) / 256), (unsigned)(s1 - s0));
+            hipLaunchKernelGGL(run_noise_kernel, ngrid, dim3(256), 0, h->stream, h->c, s0, h->noise);
+            LDPCP_HIP_TRY(hipGetLastError());
+        }
+        step<<<sgrid, sblock, 0, h->stream>>>(h->c, h->P, s0, s1, h->noise);
         LDPCP_HIP_TRY(hipGetLastError());
     }
     LDPCP_HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1337,6 +1437,7 @@ extern "C" int ldpcp_run_destroy(ldpcp_run* h)
     if (!h) return LDPCP_OK;
     if (h->c.replay) (void)hipFree((void*)h->P.g);
     if (h->frames_block) (void)hipFree(h->frames_block);
+    if (h->noise) (void)hipFree(h->noise);
     (void)hipFree(h->state_block);
     (void)hipStreamDestroy(h->stream);
     delete h;
