@@ -199,7 +199,14 @@ def serving(seconds, budget_gb, rate=20.0, seed=11, fp64_limit_s=60.0):
     srv.submit(reqs[0][1])
     while not srv.completed():
         srv.pump(1, wait=True)
-    res = S.serve(srv, reqs)
+    last = [time.perf_counter()]
+
+    def progress(k, _):
+        if time.perf_counter() - last[0] > 30.0:  # a line every ~30 s on long streams
+            last[0] = time.perf_counter()
+            print(f"[bench] serving: request {k + 1} of {len(reqs)} done", file=sys.stderr, flush=True)
+
+    res = S.serve(srv, reqs, on_done=progress)
     info = srv.info()
     srv.close()
     steps = sum(r.traj_steps for r in res["results"])

@@ -2,8 +2,11 @@
 #include "restart_file.h"
 
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <string.h>
+
+#include "ldpcp.h"
 
 namespace ldpcp_host {
 
@@ -42,6 +45,15 @@ public:
     double f64() { return get<double>(); }
     bool bytes(void* p, size_t n) { return fread(p, 1, n, f_) == n; }
     bool eof() const { return eof_; }
+    // a size_t count followed by that many doubles (StreamerWriter::WriteVector)
+    bool vec(std::vector<double>& v)
+    {
+        const uint64_t n = u64();
+        if (eof_ || n > (1ull << 36)) return eof_ = true, false;
+        v.resize(n);
+        if (n && !bytes(v.data(), n * sizeof(double))) return eof_ = true, false;
+        return true;
+    }
 
 private:
     FILE* f_;
@@ -49,7 +61,8 @@ private:
 };
 
 // The saved-field flags the runner sets on the integrator (Single.h:3261-3271):
-// x always; v, f, e, state and barrier with WRITEX; eta never (no input key).
+// x always; v, f, e, state and barrier with WRITEX, except v for the
+// overdamped integrator, which clears it (:2822-2823); eta never (no input key).
 struct SaveFlags {
     int x, v, f, e, eta, state, barrier;
 };
@@ -57,8 +70,12 @@ struct SaveFlags {
 SaveFlags save_flags(const RunnerConfig& c)
 {
     const int w = c.write_x ? 1 : 0;
-    return {1, w, w, w, 0, w, w};
+    return {1, c.overdamped ? 0 : w, w, w, 0, w, w};
 }
+
+// bRandomizeEta: the programs pass true, their input has no key for it
+// (langevinDynamicsSingleBarrier.cpp:136)
+constexpr int kRandomizeEta = 1;
 
 // m_Xc0 of the double barrier (Double.h:1135-1146): the apex of the two
 // inner slopes, written to the header and not checked on read.
@@ -76,6 +93,271 @@ bool fail(FILE* f, const char* what)
     fprintf(stderr, "Error: %s\n", what);
     fclose(f);
     return false;
+}
+
+// One lane of TrajectoryData as WriteRestartFile lays it out
+// (langevinDynamicsSingleBarrier.h:436-572).
+struct RefLane {
+    double x, v, f, e, eta, barrier;
+    std::vector<double> traj[8];  // x, v, f, pe, ke, eta, state, barrier frames
+    double state, prev_state, slope_pos, slope_neg, height, count0, count1, well, periodic, well_prev, periodic_prev;
+    double v0_instance, eta0_instance, state0_instance;
+    double tick_prev = 0.0;
+    int passed = 0;
+    std::vector<double> ticks;
+    std::vector<double> fpt, fpt2, tpt, tpt2;
+    double fpt_last = 0.0, fpt2_last = 0.0, tpt_last = 0.0, tpt2_last = 0.0;
+    int fpt_up = 0, fpt2_up = 0, tpt_up = 0, tpt2_up = 0, tpt2_well = 0, fpt2_idx[2] = {0, 0};
+    std::vector<int> cg_size;
+    std::vector<std::vector<double>> cg;
+    std::vector<double> cg_last;
+    std::vector<int> cg_up, cg_well;
+};
+
+bool read_ref_lane(Reader& r, const RunnerConfig& c, bool wt, bool fpt, RefLane& L)
+{
+    L.x = r.f64();
+    L.v = r.f64();
+    L.f = r.f64();
+    L.e = r.f64();
+    L.eta = r.f64();
+    L.barrier = r.f64();
+    for (auto& t : L.traj)
+        if (!r.vec(t)) return false;
+    L.state = r.f64();
+    L.prev_state = r.f64();
+    L.slope_pos = r.f64();
+    L.slope_neg = r.f64();
+    L.height = r.f64();
+    L.count0 = r.f64();
+    L.count1 = r.f64();
+    L.well = r.f64();
+    L.periodic = r.f64();
+    L.well_prev = r.f64();
+    L.periodic_prev = r.f64();
+    r.i32();  // entering flags: recomputed by the next step
+    r.i32();
+    L.v0_instance = r.f64();
+    L.eta0_instance = r.f64();
+    L.state0_instance = r.f64();
+    for (int k = 0; k < 4; ++k) r.f64();  // the *Stored indices of the SIMD bookkeeping
+    r.i32();
+    r.i32();
+    if (wt) {
+        L.tick_prev = r.f64();
+        L.passed = r.i32();
+        if (!r.vec(L.ticks)) return false;
+    }
+    if (fpt) {
+        std::vector<double> skip;
+        if (!r.vec(L.fpt) || !r.vec(skip) || !r.vec(skip)) return false;  // barrier logs feed no output
+        L.fpt_last = r.f64();
+        L.fpt_up = r.i32();
+        r.f64();
+        r.f64();
+        r.i32();
+        r.i32();
+        if (!r.vec(L.fpt2)) return false;
+        L.fpt2_last = r.f64();
+        L.fpt2_up = r.i32();
+        L.fpt2_idx[0] = r.i32();
+        L.fpt2_idx[1] = r.i32();
+    }
+    if (c.tpt) {
+        if (!r.vec(L.tpt)) return false;
+        L.tpt_last = r.f64();
+        L.tpt_up = r.i32();
+        if (!r.vec(L.tpt2)) return false;
+        L.tpt2_last = r.f64();
+        L.tpt2_up = r.i32();
+        L.tpt2_well = r.i32();
+    }
+    if (c.fptcg) {
+        const uint64_t n = r.u64();
+        if (r.eof() || n > 1024) return false;
+        L.cg_size.resize(n);
+        for (auto& v : L.cg_size) v = r.i32();
+        L.cg.resize(n);
+        L.cg_last.resize(n);
+        L.cg_up.resize(n);
+        L.cg_well.resize(n);
+        for (uint64_t j = 0; j < n; ++j) {
+            if (!r.vec(L.cg[j])) return false;
+            L.cg_last[j] = r.f64();
+            L.cg_up[j] = r.i32();
+            L.cg_well[j] = r.i32();
+        }
+    }
+    return !r.eof();
+}
+
+// STATISTICS::AVERAGE sums a list front to back; the device keeps the same sum as
+// it goes, so the running sum of an imported list is this loop.
+double list_sum(const std::vector<double>& v)
+{
+    double s = 0.0;
+    for (double x : v) s += x;
+    return s;
+}
+
+// The reference program's single-barrier body -> checkpoint image.
+bool import_reference_body(Reader& r, FILE* f, const RunnerConfig& c, int W, const ImageSpec& spec,
+                           uint64_t nsteps_total, std::vector<char>& image)
+{
+    if (r.i32() != kBodyStart) return fail(f, "restart file corrupted");
+    const bool wt = r.i32() == 1;
+    const bool fpt = r.i32() == 1;
+    if (wt != c.waiting) return fail(f, "inconsistent bWaitingTimeDistribution");
+    if (fpt != c.fpt) return fail(f, "inconsistent bFirstPassageTime");
+    std::vector<RefLane> lanes(W);
+    for (int i = 0; i < W; ++i)
+        if (!read_ref_lane(r, c, wt, fpt, lanes[i])) return fail(f, "restart file corrupted");
+    const uint64_t nb = r.u64();
+    if (r.eof() || nb > (1u << 24)) return fail(f, "restart file corrupted");
+    std::vector<int32_t> bstate(nb * W);
+    std::vector<uint64_t> bstep(nb * W);
+    for (uint64_t b = 0; b < nb; ++b)
+        for (int i = 0; i < W; ++i) {
+            bstate[b * W + i] = r.i32();
+            bstep[b * W + i] = r.u64();
+        }
+    const int min_id = r.i32();
+    std::vector<double> r_prev(W, 0.0);
+    if (c.overdamped)
+        for (auto& v : r_prev) v = r.f64();
+    if (r.i32() != kRestartEnd || r.eof()) return fail(f, "restart file corrupted");
+    if (fgetc(f) != EOF) return fail(f, "restart file corrupted");
+
+    // checks the resumed run relies on
+    const uint64_t n_frames = lanes[0].traj[0].size();
+    if (n_frames != nsteps_total / c.save_freq + 1)
+        return fail(f, "the restart file's frames do not match its step count "
+                       "(NUMBEROFSTEPS was not a multiple of FREQSAVE)");
+    const int nfields = spec.n_fields;
+    const int src_field[7] = {0, 1, 2, 3, 4, 6, 7};  // x v f pe ke state barrier (eta is not kept)
+    for (int i = 0; i < W; ++i) {
+        // the overdamped integrator saves no velocity frames (Single.h:2822-2823); its
+        // frames of v are zeros here, as the device records them
+        if (c.overdamped && nfields > 1 && lanes[i].traj[1].empty()) lanes[i].traj[1].assign(n_frames, 0.0);
+        for (int k = 0; k < nfields; ++k)
+            if (lanes[i].traj[src_field[k]].size() != n_frames)
+                return fail(f, "the restart file lacks frames this run saves (WRITEX)");
+        if (c.fptcg) {
+            bool same = lanes[i].cg_size.size() == c.cg_size.size();
+            for (size_t j = 0; same && j < c.cg_size.size(); ++j) same = lanes[i].cg_size[j] == c.cg_size[j];
+            if (!same) return fail(f, "inconsistent FPTCG sizes");
+        }
+    }
+
+    const uint64_t n = (uint64_t)W, wb = (uint64_t)spec.barrier_window, cap = spec.tick_capacity;
+    const uint64_t o_lanes = sizeof(ldpcp_checkpoint_header);
+    const uint64_t o_wbase = o_lanes + n * sizeof(ldpcp_lane);
+    const uint64_t o_win = o_wbase + ((n * sizeof(int32_t) + 7) & ~7ull);
+    const uint64_t o_frames = o_win + n * 2 * wb * sizeof(uint64_t);
+    const uint64_t o_ticks = o_frames + (uint64_t)nfields * n_frames * n * sizeof(double);
+    const uint64_t total = o_ticks + n * cap * sizeof(double);
+    image.assign(total, 0);
+    char* b = image.data();
+
+    ldpcp_checkpoint_header hd;
+    memset(&hd, 0, sizeof(hd));
+    hd.magic = LDPCP_CHECKPOINT_MAGIC;
+    hd.header_bytes = sizeof(ldpcp_checkpoint_header);
+    hd.lane_bytes = sizeof(ldpcp_lane);
+    hd.n_traj = (uint32_t)n;
+    hd.traj_offset = spec.traj_offset;
+    hd.model = c.model;
+    hd.overdamped = c.overdamped ? 1 : 0;
+    hd.barrier_window = spec.barrier_window;
+    hd.n_fields = nfields;
+    hd.seed = spec.seed;
+    hd.steps_done = nsteps_total;
+    hd.n_frames = n_frames;
+    hd.save_freq = c.save_freq;
+    hd.tick_capacity = cap;
+    hd.timestep = c.timestep;
+    hd.bytes = total;
+    memcpy(b, &hd, sizeof(hd));
+
+    constexpr uint64_t kValid = 1ull << 63, kState = 1ull << 62, kStepMask = kState - 1;
+    for (int i = 0; i < W; ++i) {
+        const RefLane& R = lanes[i];
+        ldpcp_lane L;
+        memset(&L, 0, sizeof(L));
+        L.x = R.x;
+        L.v = R.v;
+        L.f = R.f;
+        L.e = R.e;
+        L.r_prev = r_prev[i];
+        L.state = R.state;
+        L.barrier = R.barrier;
+        L.state_count[0] = R.count0;
+        L.state_count[1] = R.count1;
+        L.well = R.well;
+        L.periodic = R.periodic;
+        L.v0_instance = R.v0_instance;
+        L.state0_instance = R.state0_instance;
+        L.fpt_sum = list_sum(R.fpt);
+        L.fpt_n = R.fpt.size();
+        L.fpt_last_tick = R.fpt_last;
+        L.fpt_to_update = R.fpt_up;
+        L.fpt2_sum = list_sum(R.fpt2);
+        L.fpt2_n = R.fpt2.size();
+        L.fpt2_last_tick = R.fpt2_last;
+        L.fpt2_to_update = R.fpt2_up;
+        L.fpt2_idx[0] = R.fpt2_idx[0];
+        L.fpt2_idx[1] = R.fpt2_idx[1];
+        L.tpt_sum = list_sum(R.tpt);
+        L.tpt_n = R.tpt.size();
+        L.tpt_last_tick = R.tpt_last;
+        L.tpt_to_update = R.tpt_up;
+        L.tpt2_sum = list_sum(R.tpt2);
+        L.tpt2_n = R.tpt2.size();
+        L.tpt2_last_tick = R.tpt2_last;
+        L.tpt2_to_update = R.tpt2_up;
+        L.tpt2_initial_well = R.tpt2_well;
+        for (size_t j = 0; j < R.cg.size() && j < LDPCP_MAX_CG; ++j) {
+            L.cg_sum[j] = list_sum(R.cg[j]);
+            L.cg_n[j] = R.cg[j].size();
+            L.cg_last_tick[j] = R.cg_last[j];
+            L.cg_to_update[j] = R.cg_up[j];
+            L.cg_initial_well[j] = R.cg_well[j];
+        }
+        // waiting time: the barrier ticks and the moments of their spacings, in
+        // the order the device accumulates them
+        L.wt_tick_prev = R.tick_prev;
+        L.wt_passed = R.passed;
+        L.wt_ticks = R.ticks.size();
+        for (size_t k = 0; k < R.ticks.size(); ++k) {
+            if (k > 0) {
+                const double w = (R.ticks[k] - L.wt_last) * c.timestep;
+                L.wt_sum += w;
+                L.wt_sum2 = fma(w, w, L.wt_sum2);
+                if (L.wt_n == 0 || w > L.wt_max) L.wt_max = w;
+                L.wt_n++;
+            }
+            L.wt_last = R.ticks[k];
+        }
+        L.steps_done = nsteps_total;
+        memcpy(b + o_lanes + i * sizeof(ldpcp_lane), &L, sizeof(L));
+        // barrier table -> the lane's window, centred on its current barrier
+        const int32_t base = (int32_t)R.periodic - spec.barrier_window / 2;
+        memcpy(b + o_wbase + i * sizeof(int32_t), &base, sizeof(base));
+        for (uint64_t k = 0; k < nb; ++k) {
+            const int64_t slot = (int64_t)min_id + (int64_t)k - base;
+            if (slot < 0 || slot >= (int64_t)wb) continue;  // beyond reach of the window
+            const uint64_t e = kValid | (bstate[k * W + i] ? kState : 0ull) | (bstep[k * W + i] & kStepMask);
+            memcpy(b + o_win + (((uint64_t)i * 2) * wb + (uint64_t)slot) * sizeof(uint64_t), &e, sizeof(e));
+        }
+        for (int fld = 0; fld < nfields; ++fld)
+            for (uint64_t fr = 0; fr < n_frames; ++fr)
+                memcpy(b + o_frames + (((uint64_t)fld * n_frames + fr) * n + i) * sizeof(double),
+                       &R.traj[src_field[fld]][fr], sizeof(double));
+        const uint64_t keep = R.ticks.size() < cap ? R.ticks.size() : cap;
+        if (keep) memcpy(b + o_ticks + (uint64_t)i * cap * sizeof(double), R.ticks.data(), keep * sizeof(double));
+    }
+    fclose(f);
+    return true;
 }
 
 }  // namespace
@@ -116,7 +398,7 @@ bool write_restart_file(const RunnerConfig& c, int simd_width, int rank, int tri
     w.i32(sf.f);
     w.i32(sf.e);
     w.i32(c.randomize_v ? 1 : 0);
-    w.i32(0);  // RandomizeEta: no input key, always false
+    w.i32(kRandomizeEta);
     w.f64(c.periodic_length);
     if (c.model == 0) {
         w.f64(c.height[0][0]);
@@ -161,8 +443,8 @@ bool write_restart_file(const RunnerConfig& c, int simd_width, int rank, int tri
     return true;
 }
 
-bool read_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, std::vector<char>& image,
-                       uint64_t* nsteps_total)
+bool read_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, const ImageSpec& spec,
+                       std::vector<char>& image, uint64_t* nsteps_total)
 {
     const std::string name = restart_file_name(c.input_restart, rank, trial);
     FILE* f = fopen(name.c_str(), "rb");
@@ -173,7 +455,11 @@ bool read_restart_file(const RunnerConfig& c, int simd_width, int rank, int tria
     Reader r(f);
     const SaveFlags sf = save_flags(c);
     // the reference's checks, in its order and words (Single.h:1143-1190; Double.h:1302-1350)
-    if (r.i32() != kRestartSignature) return fail(f, "inconsistent signature");
+    const int signature = r.i32();
+    const bool reference_file = (signature == kReferenceSignature);
+    if (signature != kRestartSignature && !reference_file) return fail(f, "inconsistent signature");
+    if (reference_file && c.model != 0)
+        return fail(f, "restart files of the reference's double-barrier program are not read by this build");
     if (r.i32() != simd_width) return fail(f, "inconsistent simd witdth");
     if (r.f64() != c.x0) return fail(f, "inconsistent x0");
     if (r.f64() != c.v0) return fail(f, "inconsistent v0");
@@ -191,7 +477,7 @@ bool read_restart_file(const RunnerConfig& c, int simd_width, int rank, int tria
     if (r.i32() != sf.f) return fail(f, "inconsistent bSaveF");
     if (r.i32() != sf.e) return fail(f, "inconsistent bSaveE");
     if (r.i32() != (c.randomize_v ? 1 : 0)) return fail(f, "inconsistent randomize velocity");
-    if (r.i32() != 0) return fail(f, "inconsistent randomize eta");
+    if (r.i32() != kRandomizeEta) return fail(f, "inconsistent randomize eta");
     if (r.f64() != c.periodic_length) return fail(f, "inconsistent periodic length0");
     if (c.model == 0) {
         if (r.f64() != c.height[0][0]) return fail(f, "inconsistent barrier height0 [0]");
@@ -216,10 +502,11 @@ bool read_restart_file(const RunnerConfig& c, int simd_width, int rank, int tria
     if (r.i32() != sf.state) return fail(f, "inconsistent bSaveState");
     if (r.i32() != sf.barrier) return fail(f, "inconsistent save barrier");
     if (r.i32() != rank) return fail(f, "inconsistent rank");
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 3; ++k) {  // noise positions: the step count locates this program's noise
         r.get<int64_t>();
         r.u64();
     }
+    if (reference_file) return import_reference_body(r, f, c, simd_width, spec, *nsteps_total, image);
     if (r.i32() != kBodyStart) return fail(f, "restart file corrupted");
     const uint64_t n = r.u64();
     if (r.eof() || n > (1ull << 40)) return fail(f, "restart file corrupted");

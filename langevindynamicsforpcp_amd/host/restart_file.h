@@ -8,8 +8,9 @@
 // its read-side consistency checks with the same messages.  The body after the
 // header is the library's checkpoint image (ldpcp_checkpoint_header + device
 // state, include/ldpcp.h) in place of the reference's per-lane event lists and
-// MKL bank positions, so the signature differs from the reference's 2345: each
-// program rejects the other's file at its first check.
+// MKL bank positions, so the signature differs from the reference's 2345 and
+// the reference program refuses these files at its first check.  This program
+// reads both kinds (read_restart_file).
 #pragma once
 #include <stdint.h>
 
@@ -30,10 +31,29 @@ std::string restart_file_name(const std::string& base, int rank, int trial);
 bool write_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, uint64_t nsteps_total,
                         const std::vector<char>& image);
 
+// What a checkpoint image built from a reference file must say about the
+// handle that resumes it (ldpcp_run_resume checks these).
+struct ImageSpec {
+    uint64_t seed = 0;
+    uint32_t traj_offset = 0;
+    int32_t barrier_window = 0;
+    int32_t n_fields = 1;
+    uint64_t tick_capacity = 0;
+};
+
 // ReadRestartFile: false (after the reference's message on stderr) when the
 // file is missing or any setting differs from c.  On success image holds the
-// checkpoint image and nsteps_total the header's step count.
-bool read_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, std::vector<char>& image,
-                       uint64_t* nsteps_total);
+// checkpoint image and nsteps_total the header's step count.  Besides this
+// program's files it reads the reference program's own single-barrier files
+// (signature 2345; body TrajectoryData::WriteRestartFile,
+// langevinDynamicsSingleBarrier.h:385-576, then the barrier table, DataLocator.h:163-169,
+// and the overdamped R_prev, :2776-2785): their per-lane state, event lists
+// (as running sums and counts), frames and barrier table become a checkpoint
+// image for `spec`, so a reference run continues here (with this program's
+// Philox noise from the file's step count on).
+bool read_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, const ImageSpec& spec,
+                       std::vector<char>& image, uint64_t* nsteps_total);
+
+constexpr int kReferenceSignature = 2345;
 
 }  // namespace ldpcp_host

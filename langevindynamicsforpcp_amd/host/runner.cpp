@@ -380,7 +380,13 @@ int run(const RunnerConfig& cfg)
             if (cfg.read_restart) {
                 // PreIntegrate -> ReadRestartFile (Single.h:1346-1351); a failure ends the
                 // run as the runner's integrator-failure check does (:3436-3446)
-                bool ok = read_restart_file(cfg, W, r, i, image, &nsteps_before);
+                ImageSpec spec;  // what a reference program's file must become for this handle
+                spec.seed = p.seed;
+                spec.traj_offset = (uint32_t)((uint64_t)i * W);
+                spec.barrier_window = p.barrier_window;
+                spec.n_fields = cfg.write_x ? 7 : 1;
+                spec.tick_capacity = p.tick_capacity;
+                bool ok = read_restart_file(cfg, W, r, i, spec, image, &nsteps_before);
                 if (ok && ldpcp_run_resume(h, image.data(), image.size())) {
                     fprintf(stderr, "Error: %s\n", ldpcp_last_error());
                     ok = false;

@@ -98,7 +98,8 @@ def decode(path, overdamped=True, tpt=True, fptcg=True):
         for k in ("v0_instance", "eta0_instance", "state0_instance", "well_stored", "periodic_stored",
                   "well_prev_stored", "periodic_prev_stored"):
             L[k] = r.f64()
-        r.i32(); r.i32()
+        L["entering_well_stored"] = r.i32()
+        L["entering_period_stored"] = r.i32()
         if b_wt:
             L["tick_prev"] = r.f64()
             L["barrier_passed"] = r.i32()
@@ -146,3 +147,127 @@ def decode(path, overdamped=True, tpt=True, fptcg=True):
     assert r.pos == len(r.buf), (r.pos, len(r.buf))
     return {"header": h, "lanes": lanes, "barrier_state": states, "barrier_step": steps,
             "barrier_min_id": min_id, "r_prev": tail}
+
+
+class _Writer:
+    def __init__(self):
+        self.parts = []
+
+    def i32(self, v):
+        self.parts.append(struct.pack("<i", int(v)))
+
+    def u64(self, v):
+        self.parts.append(struct.pack("<Q", int(v)))
+
+    def i64(self, v):
+        self.parts.append(struct.pack("<q", int(v)))
+
+    def f64(self, v):
+        self.parts.append(struct.pack("<d", float(v)))
+
+    def vec(self, a, dtype="<f8"):
+        self.parts.append(np.ascontiguousarray(a, dtype=dtype).tobytes())
+
+    def sized_vec(self, a):
+        self.u64(len(a))
+        self.vec(a)
+
+
+def encode(d, path, overdamped=True, tpt=True, fptcg=True):
+    """The inverse of decode for the single-barrier layout: writes the dict
+    decode returns (header, lanes, barrier table, R_prev) as a restart file of
+    the reference program, for tests that need such a file on a machine
+    without the reference's own files."""
+    w = _Writer()
+    h = d["header"]
+    w.i32(h["signature"])
+    w.i32(h["simd_width"])
+    for k in ("x0", "v0", "gamma", "kb", "temperature", "mass", "timestep"):
+        w.f64(h[k])
+    w.u64(h["nsteps_total"])
+    w.i32(h["waiting_time"])
+    w.u64(h["stat_freq"])
+    w.u64(h["save_freq"])
+    for v in h["save_flags"]:
+        w.i32(v)
+    w.i32(h["randomize_v"])
+    w.i32(h["randomize_eta"])
+    for k in ("periodic_length", "h0", "h1", "initial_state"):
+        w.f64(h[k])
+    w.i32(h["randomize_state"])
+    w.i32(h["same_initial_barrier"])
+    for k in ("q", "tau", "eta0"):
+        w.f64(h[k])
+    for v in h["save_flags2"]:
+        w.i32(v)
+    w.i32(h["rank"])
+    for renew, offset in h["rng"]:
+        w.i64(renew)
+        w.u64(offset)
+    w.i32(9998)
+    b_wt = int(h["waiting_time"])
+    b_fpt = int(d.get("b_fpt", 1))
+    w.i32(b_wt)
+    w.i32(b_fpt)
+    for L in d["lanes"]:
+        for k in ("x", "v", "f", "e", "eta", "barrier"):
+            w.f64(L[k])
+        for k in ("traj_x", "traj_v", "traj_f", "traj_pe", "traj_ke", "traj_eta", "traj_state", "traj_barrier"):
+            w.sized_vec(L[k])
+        for k in ("current_state", "previous_state", "slope_pos", "slope_neg", "current_height",
+                  "state_count0", "state_count1", "well", "periodic", "well_prev", "periodic_prev"):
+            w.f64(L[k])
+        w.i32(L["entering_well"])
+        w.i32(L["entering_period"])
+        for k in ("v0_instance", "eta0_instance", "state0_instance", "well_stored", "periodic_stored",
+                  "well_prev_stored", "periodic_prev_stored"):
+            w.f64(L[k])
+        w.i32(L.get("entering_well_stored", 0))
+        w.i32(L.get("entering_period_stored", 0))
+        if b_wt:
+            w.f64(L["tick_prev"])
+            w.i32(L["barrier_passed"])
+            w.sized_vec(L["barrier_ticks"])
+        if b_fpt:
+            w.sized_vec(L["fpt"])
+            w.sized_vec(L["instant_barrier"])
+            w.sized_vec(L["initial_barrier"])
+            w.f64(L["fpt_last_tick"])
+            w.i32(L["fpt_to_update"])
+            w.f64(L["fpt_init_state"][0])
+            w.f64(L["fpt_init_state"][1])
+            w.i32(L["fpt_init_index"][0])
+            w.i32(L["fpt_init_index"][1])
+            w.sized_vec(L["fpt2"])
+            w.f64(L["fpt2_last_tick"])
+            w.i32(L["fpt2_to_update"])
+            w.i32(L["fpt2_index"][0])
+            w.i32(L["fpt2_index"][1])
+        if tpt:
+            w.sized_vec(L["tpt"])
+            w.f64(L["tpt_last_tick"])
+            w.i32(L["tpt_to_update"])
+            w.sized_vec(L["tpt2"])
+            w.f64(L["tpt2_last_tick"])
+            w.i32(L["tpt2_to_update"])
+            w.i32(L["tpt2_initial_well"])
+        if fptcg:
+            w.u64(len(L["cg_sizes"]))
+            w.vec(L["cg_sizes"], "<i4")
+            for cg in L["cg"]:
+                w.sized_vec(cg["list"])
+                w.f64(cg["last_tick"])
+                w.i32(cg["to_update"])
+                w.i32(cg["initial_well"])
+    states, steps = d["barrier_state"], d["barrier_step"]
+    w.u64(states.shape[0])
+    for b in range(states.shape[0]):
+        for i in range(states.shape[1]):
+            w.i32(states[b, i])
+            w.u64(steps[b, i])
+    w.i32(d["barrier_min_id"])
+    if overdamped:
+        w.vec(d["r_prev"])
+    w.i32(9990)
+    with open(path, "wb") as f:
+        f.write(b"".join(w.parts))

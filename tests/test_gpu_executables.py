@@ -342,3 +342,103 @@ def test_executable_restart_checks_settings(tmp_path):
                        text=True, timeout=300)
     assert r.returncode == 0
     assert "Error: inconsistent time step0" in r.stderr and "Error: can't read restart file on rank" in r.stderr
+
+
+def test_executable_continues_a_reference_restart(tmp_path):
+    """READRESTART of the reference program's own restart files: the quick
+    test's state after 3e9 steps (4 ranks x 4 lanes: positions, states, state
+    counts, barrier tables, R_prev and all 10001 x frames, from the fixtures)
+    with event lists drawn here, written in the reference's layout.  With
+    NUMBEROFSTEPS 0 every output is the imported state's: the per-lane means of
+    the lists, the reference's own msd.log and stateCount.log; with 600000 more
+    steps the run continues from it (state counts grow by exactly the steps)."""
+    import sys
+    sys.path.insert(0, GOLD)
+    import restart_decode as R
+    st = json.load(open(os.path.join(GOLD, "quick_test_state.json")))
+    frames_x = np.load(os.path.join(GOLD, "quick_test_frames.npz"))["x"]
+    rng = np.random.default_rng(5)
+    cg_sizes = [2, 3, 4, 5, 10, 100, 1000]
+
+    def seq_mean(v):
+        s = 0.0
+        for x in v:
+            s += x
+        return s / len(v) if len(v) else 0.0
+
+    expect = {k: [] for k in ("fpt", "fpt2", "tpt", "tpt2")}
+    expect_cg = [[] for _ in cg_sizes]
+    spacings = []
+    for rank in range(4):
+        rk = st["ranks"][rank]
+        lanes = []
+        for i in range(4):
+            w = st["lanes"][rank * 4 + i]
+            L = {"x": w["x"], "v": 0.0, "f": w["f"], "e": w["e"], "eta": 0.0,
+                 "barrier": 7.2 if w["state"] == 0.0 else 2.0,
+                 "traj_x": frames_x[rank * 4 + i], "current_state": w["state"], "previous_state": w["state"],
+                 "slope_pos": 0.0, "slope_neg": 0.0, "current_height": 0.0,
+                 "state_count0": w["state_count"][0], "state_count1": w["state_count"][1],
+                 "well": w["well"], "periodic": w["periodic"], "well_prev": w["well"], "periodic_prev": w["periodic"],
+                 "entering_well": 0, "entering_period": 0, "v0_instance": w["v0_instance"], "eta0_instance": 0.0,
+                 "state0_instance": w["state0_instance"], "well_stored": 0.0, "periodic_stored": 0.0,
+                 "well_prev_stored": 0.0, "periodic_prev_stored": 0.0,
+                 "tick_prev": 0.0, "barrier_passed": 0,
+                 "barrier_ticks": np.sort(rng.integers(0, 3 * 10**9, w["ticks_n"])).astype(np.float64),
+                 "instant_barrier": [], "initial_barrier": [], "fpt_last_tick": w["fpt_last_tick"],
+                 "fpt_to_update": w["fpt_to_update"], "fpt_init_state": (0.0, 0.0), "fpt_init_index": (0, 0),
+                 "fpt2_last_tick": w["fpt2_last_tick"], "fpt2_to_update": 0, "fpt2_index": tuple(w["fpt2_index"]),
+                 "tpt_last_tick": 0.0, "tpt_to_update": 1, "tpt2_last_tick": 0.0, "tpt2_to_update": 1,
+                 "tpt2_initial_well": int(w["well"]), "cg_sizes": cg_sizes, "cg": []}
+            for k in ("traj_v", "traj_f", "traj_pe", "traj_ke", "traj_eta", "traj_state", "traj_barrier"):
+                L[k] = []
+            for k in ("fpt", "fpt2", "tpt", "tpt2"):
+                L[k] = rng.exponential(40.0, w[k + "_n"])
+                expect[k].append(seq_mean(L[k]))
+            for j, n in enumerate(w["cg_n"]):
+                lst = rng.exponential(300.0, n)
+                L["cg"].append({"list": lst, "last_tick": 0.0, "to_update": 0, "initial_well": int(w["well"])})
+                expect_cg[j].append(seq_mean(lst))
+            spacings += list(np.diff(L["barrier_ticks"]) * 1e-4)
+            lanes.append(L)
+        header = {"signature": 2345, "simd_width": 4, "x0": -2.9, "v0": 1.0, "gamma": 1.0, "kb": 1.0,
+                  "temperature": 1.0, "mass": 1.0, "timestep": 1e-4, "nsteps_total": 3 * 10**9, "waiting_time": 1,
+                  "stat_freq": 1, "save_freq": 300000, "save_flags": [1, 0, 0, 0], "randomize_v": 1,
+                  "randomize_eta": 1, "periodic_length": 5.8, "h0": 7.2, "h1": 2.0, "initial_state": 0.0,
+                  "randomize_state": 1, "same_initial_barrier": 0, "q": 0.2, "tau": 1e-5, "eta0": 0.0,
+                  "save_flags2": [0, 0, 0], "rank": rank, "rng": [(0, 0)] * 3}
+        R.encode({"header": header, "lanes": lanes, "barrier_state": np.array(rk["barrier_state"], np.int32),
+                  "barrier_step": np.array(rk["barrier_step"], np.uint64), "barrier_min_id": rk["barrier_min_id"],
+                  "r_prev": np.array(rk["r_prev"])}, str(tmp_path / f"ref_{rank:05d}_00000.restart"))
+
+    base = open(os.path.join(GOLD, "quick_test.inp")).read()
+    for k, v in dict(OUTPUTBASE="out", WRITEX="false", READRESTART="true", INPUTRESTART="ref",
+                     WRITERESTART="false", NUMBEROFPREVIOUSSTEPS="3000000000").items():
+        base = re.sub(rf"(?m)^{k} .*$", f"{k} {v}", base)
+    exe = os.path.join(BIN, "langevinDynamicsSingleBarrier")
+
+    def run(nsteps):
+        (tmp_path / "run.inp").write_text(re.sub(r"(?m)^NUMBEROFSTEPS .*$", f"NUMBEROFSTEPS {nsteps}", base))
+        print(f"[gpu exe reference restart] NUMBEROFSTEPS {nsteps}", flush=True)
+        r = subprocess.run([exe, "-i", "run.inp", "--ranks", "4", "--simd-width", "4"], cwd=tmp_path,
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0 and "Error" not in r.stderr, r.stderr[-2000:]
+
+    run(0)
+    for k, name in (("fpt", "firstPassageTime"), ("fpt2", "firstPassageTime2"), ("tpt", "transitionPathTime"),
+                    ("tpt2", "transitionPathTime2")):
+        assert np.fromfile(tmp_path / f"out_{name}.dat", "<f8").tolist() == expect[k], name
+    for j, size in enumerate(cg_sizes):
+        got = np.fromfile(tmp_path / f"out_firstPassageTimeCG2_{j:04d}_{size:04d}.dat", "<f8")
+        assert got.tolist() == expect_cg[j], size
+    msd = np.loadtxt(tmp_path / "out_msd.log", comments="#")
+    np.testing.assert_allclose(msd, np.array(st["outputs"]["msd.log"]), rtol=2e-7, atol=0)
+    sc = [ln for ln in (tmp_path / "out_stateCount.log").read_text().splitlines() if ln.startswith("# state")]
+    assert [ln.split() for ln in sc] == [ln.split() for ln in st["outputs"]["stateCount.log"]]
+    wt = [ln for ln in (tmp_path / "out_waitingTime.log").read_text().splitlines() if ln.startswith("# Nwt")]
+    assert int(wt[0].split()[-1]) == len(spacings)
+
+    run(600000)
+    total = sum(sum(int(x) for x in ln.split()[3:4]) for ln in
+                [ln for ln in (tmp_path / "out_stateCount.log").read_text().splitlines() if ln.startswith("# state")])
+    assert total == sum(int(ln.split()[3]) for ln in st["outputs"]["stateCount.log"]) + 16 * 600000
