@@ -102,6 +102,14 @@ int ldpcp_run_set_replay(ldpcp_run* h, const double* gauss, uint64_t n_gauss,
  * FREQSAVE >= 10), frames every FREQSAVE steps, statistics every step. */
 int ldpcp_run_integrate(ldpcp_run* h, uint64_t steps_per_launch);
 
+/* ldpcp_run_integrate in two halves: ldpcp_run_launch enqueues the run on the
+ * handle's own stream and returns; ldpcp_run_wait waits for it and reports
+ * per-trajectory errors.  Handles launched one after another run concurrently
+ * on the GPU (the reference's MPI ranks integrate at the same time,
+ * langevinDynamicsSingleBarrier.h:3429-3446).  Reading results waits first. */
+int ldpcp_run_launch(ldpcp_run* h, uint64_t steps_per_launch);
+int ldpcp_run_wait(ldpcp_run* h);
+
 /* Per-trajectory results for trajectories [begin, begin+n). */
 int ldpcp_run_lanes(ldpcp_run* h, uint32_t begin, uint32_t n, ldpcp_lane* out);
 

@@ -129,6 +129,8 @@ _SIGNATURES = [
                                        C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_int8), C.c_int32,
                                        C.c_int32]),
     ("ldpcp_run_integrate", C.c_int, [_P, C.c_uint64]),
+    ("ldpcp_run_launch", C.c_int, [_P, C.c_uint64]),
+    ("ldpcp_run_wait", C.c_int, [_P]),
     ("ldpcp_run_lanes", C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(LaneResult)]),
     ("ldpcp_run_n_frames", C.c_uint64, [_P]),
     ("ldpcp_run_frames", C.c_int, [_P, C.c_int32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),

@@ -897,6 +897,7 @@ struct ldpcp_run {
     uint64_t step_base;   // absolute step the next integrate starts after
     int phase;            // 0 fresh, 1 resumed (next integrate continues), 2 resumed run done
     bool integrated;
+    bool pending;         // launched, not yet waited for
 };
 
 using namespace ldpcp;
@@ -1015,6 +1016,10 @@ extern "C" int ldpcp_run_set_replay(ldpcp_run* h, const double* gauss, uint64_t 
                                     uint64_t n_flip, const double* catchup, uint64_t catch_cap, const int8_t* binit,
                                     int32_t bmin, int32_t bn)
 {
+    if (h && h->pending) {
+        set_error("ldpcp_run_set_replay: the handle has a launched run; wait for it first");
+        return LDPCP_EINVAL;
+    }
     if (!h || !gauss || (!flip && n_flip) || (!catchup && catch_cap) || (!binit && bn)) {
         set_error("ldpcp_run_set_replay: null argument");
         return LDPCP_EINVAL;
@@ -1058,14 +1063,18 @@ extern "C" int ldpcp_run_set_replay(ldpcp_run* h, const double* gauss, uint64_t 
     return LDPCP_OK;
 }
 
-extern "C" int ldpcp_run_integrate(ldpcp_run* h, uint64_t steps_per_launch)
+extern "C" int ldpcp_run_launch(ldpcp_run* h, uint64_t steps_per_launch)
 {
     if (!h) {
-        set_error("ldpcp_run_integrate: null handle");
+        set_error("ldpcp_run_launch: null handle");
+        return LDPCP_EINVAL;
+    }
+    if (h->pending) {
+        set_error("ldpcp_run_launch: the handle's previous run has not been waited for");
         return LDPCP_EINVAL;
     }
     if (h->phase == 2) {
-        set_error("ldpcp_run_integrate: this handle already continued a checkpoint; create a new handle");
+        set_error("ldpcp_run_launch: this handle already continued a checkpoint; create a new handle");
         return LDPCP_EINVAL;
     }
     if (steps_per_launch == 0) steps_per_launch = 1ull << 20;
@@ -1100,9 +1109,21 @@ extern "C" int ldpcp_run_integrate(ldpcp_run* h, uint64_t steps_per_launch)
         step<<<sgrid, sblock, 0, h->stream>>>(h->c, h->P, s0, s1, h->noise);
         LDPCP_HIP_TRY(hipGetLastError());
     }
+    h->pending = true;
+    if (h->phase == 1) h->phase = 2;
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_wait(ldpcp_run* h)
+{
+    if (!h || !h->pending) {
+        set_error("ldpcp_run_wait: null handle or nothing launched");
+        return LDPCP_EINVAL;
+    }
+    const uint64_t n = h->c.n_traj;
+    h->pending = false;
     LDPCP_HIP_TRY(hipStreamSynchronize(h->stream));
     h->integrated = true;
-    if (h->phase == 1) h->phase = 2;
     // surface the first per-lane error
     std::vector<ldpcp_lane> lanes(n);
     LDPCP_HIP_TRY(hipMemcpy(lanes.data(), h->P.lanes, n * sizeof(ldpcp_lane), hipMemcpyDeviceToHost));
@@ -1118,8 +1139,18 @@ extern "C" int ldpcp_run_integrate(ldpcp_run* h, uint64_t steps_per_launch)
     return LDPCP_OK;
 }
 
+extern "C" int ldpcp_run_integrate(ldpcp_run* h, uint64_t steps_per_launch)
+{
+    const int rc = ldpcp_run_launch(h, steps_per_launch);
+    return rc ? rc : ldpcp_run_wait(h);
+}
+
 extern "C" int ldpcp_run_lanes(ldpcp_run* h, uint32_t begin, uint32_t n, ldpcp_lane* out)
 {
+    if (h && h->pending) {  // results of a launched run: wait for it first
+        const int rc = ldpcp_run_wait(h);
+        if (rc) return rc;
+    }
     if (!h || !out || (uint64_t)begin + n > h->c.n_traj) {
         set_error("ldpcp_run_lanes: bad handle, output or range");
         return LDPCP_EINVAL;
@@ -1152,6 +1183,10 @@ extern "C" uint64_t ldpcp_run_n_frames(const ldpcp_run* h) { return h ? h->n_fra
 
 extern "C" int ldpcp_run_frames(ldpcp_run* h, int32_t field, uint32_t begin, uint32_t n, double* out)
 {
+    if (h && h->pending) {  // results of a launched run: wait for it first
+        const int rc = ldpcp_run_wait(h);
+        if (rc) return rc;
+    }
     if (!h || !out || (uint64_t)begin + n > h->c.n_traj || field < 0 || field >= kFields ||
         (field > 0 && !h->params.save_all)) {
         set_error("ldpcp_run_frames: bad handle, field or range");
@@ -1168,6 +1203,10 @@ extern "C" int ldpcp_run_frames(ldpcp_run* h, int32_t field, uint32_t begin, uin
 
 extern "C" int ldpcp_run_ticks(ldpcp_run* h, uint32_t begin, uint32_t n, double* out)
 {
+    if (h && h->pending) {  // results of a launched run: wait for it first
+        const int rc = ldpcp_run_wait(h);
+        if (rc) return rc;
+    }
     if (!h || !out || (uint64_t)begin + n > h->c.n_traj || h->c.tick_cap == 0) {
         set_error("ldpcp_run_ticks: bad handle, range, or no tick capacity");
         return LDPCP_EINVAL;
@@ -1179,6 +1218,10 @@ extern "C" int ldpcp_run_ticks(ldpcp_run* h, uint32_t begin, uint32_t n, double*
 
 extern "C" int ldpcp_run_msd(ldpcp_run* h, const uint64_t* lags, uint32_t nlags, double* out)
 {
+    if (h && h->pending) {  // results of a launched run: wait for it first
+        const int rc = ldpcp_run_wait(h);
+        if (rc) return rc;
+    }
     if (!h || !lags || !out || nlags == 0 || nlags > 65535) {
         set_error("ldpcp_run_msd: bad arguments");
         return LDPCP_EINVAL;
@@ -1245,6 +1288,10 @@ CkptLayout run_layout(const ldpcp_run* h)
 
 extern "C" int ldpcp_run_set_barrier_default(ldpcp_run* h, const double* state)
 {
+    if (h && h->pending) {
+        set_error("ldpcp_run_set_barrier_default: the handle has a launched run; wait for it first");
+        return LDPCP_EINVAL;
+    }
     if (!h || !state) {
         set_error("ldpcp_run_set_barrier_default: null argument");
         return LDPCP_EINVAL;
@@ -1270,6 +1317,10 @@ extern "C" int ldpcp_run_checkpoint_size(ldpcp_run* h, uint64_t* bytes)
 
 extern "C" int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes)
 {
+    if (h && h->pending) {  // results of a launched run: wait for it first
+        const int rc = ldpcp_run_wait(h);
+        if (rc) return rc;
+    }
     if (!h || !buf) {
         set_error("ldpcp_run_checkpoint: null argument");
         return LDPCP_EINVAL;
@@ -1328,6 +1379,10 @@ extern "C" int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes)
 
 extern "C" int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes)
 {
+    if (h && h->pending) {
+        set_error("ldpcp_run_resume: the handle has a launched run; wait for it first");
+        return LDPCP_EINVAL;
+    }
     if (!h || !buf || bytes < sizeof(ldpcp_checkpoint_header)) {
         set_error("ldpcp_run_resume: null argument or truncated image");
         return LDPCP_EINVAL;
@@ -1435,6 +1490,7 @@ extern "C" int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes)
 extern "C" int ldpcp_run_destroy(ldpcp_run* h)
 {
     if (!h) return LDPCP_OK;
+    if (h->pending) (void)hipStreamSynchronize(h->stream);  // no buffer is freed under a running kernel
     if (h->c.replay) (void)hipFree((void*)h->P.g);
     if (h->frames_block) (void)hipFree(h->frames_block);
     if (h->noise) (void)hipFree(h->noise);

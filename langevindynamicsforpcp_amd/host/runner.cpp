@@ -362,21 +362,24 @@ int run(const RunnerConfig& cfg)
     // UserData::m_bInitialValue (Single.h:1378-1380)
     std::vector<std::vector<double>> state0_prev(R, std::vector<double>(W, 0.0));
     std::vector<char> image;
-    uint64_t nsteps_before = 0;  // m_NumberOfStepDonePreviously
 
     for (int i = 0; i < cfg.ntrial; ++i) {
         fprintf(stderr, "##################################################################\n");
         fprintf(stderr, "Trial %10d/%10d\n", i, cfg.ntrial);
         fprintf(stderr, "##################################################################\n");
         fprintf(stderr, "   Integrate\n");
+        // every rank's integrator is launched before any is waited for: the ranks
+        // run concurrently on the GPU, as the reference's MPI ranks do
+        const auto t0 = clock::now();
+        std::vector<ldpcp_run*> handles(R, nullptr);
+        std::vector<uint64_t> before(R, 0);
         for (int r = 0; r < R; ++r) {
-            auto t0 = clock::now();
             p.seed = (uint64_t)(uint32_t)seeds[r];
             ldpcp_run* h = nullptr;
             if (ldpcp_run_create(&p, (uint32_t)W, (uint32_t)((uint64_t)i * W), &h)) die("ldpcp_run_create");
             if (cfg.same_initial_barrier && ldpcp_run_set_barrier_default(h, state0_prev[r].data()))
                 die("ldpcp_run_set_barrier_default");
-            nsteps_before = 0;
+            uint64_t nsteps_before = 0;  // m_NumberOfStepDonePreviously
             if (cfg.read_restart) {
                 // PreIntegrate -> ReadRestartFile (Single.h:1346-1351); a failure ends the
                 // run as the runner's integrator-failure check does (:3436-3446)
@@ -395,10 +398,21 @@ int run(const RunnerConfig& cfg)
                     fprintf(stderr, "Error: can't read restart file on rank %10d\n", r);
                     fprintf(stderr, "Error: Integrator failure\n");
                     ldpcp_run_destroy(h);
+                    for (int q = 0; q < r; ++q) ldpcp_run_destroy(handles[q]);  // waits for their runs
                     exit(0);
                 }
             }
-            if (ldpcp_run_integrate(h, cfg.steps_per_launch)) die("ldpcp_run_integrate");
+            if (ldpcp_run_launch(h, cfg.steps_per_launch)) die("ldpcp_run_launch");
+            handles[r] = h;
+            before[r] = nsteps_before;
+        }
+        for (int r = 0; r < R; ++r)
+            if (ldpcp_run_wait(handles[r])) die("ldpcp_run_integrate");
+        const auto t1 = clock::now();
+        t_integrate += std::chrono::duration<double>(t1 - t0).count();
+        for (int r = 0; r < R; ++r) {
+            ldpcp_run* h = handles[r];
+            const uint64_t nsteps_before = before[r];
             if (cfg.write_restart) {
                 uint64_t bytes = 0;
                 if (ldpcp_run_checkpoint_size(h, &bytes)) die("ldpcp_run_checkpoint_size");
@@ -407,8 +421,7 @@ int run(const RunnerConfig& cfg)
                 if (!write_restart_file(cfg, W, r, i, nsteps_before + cfg.nsteps, image))
                     fprintf(stderr, "Error: can't open restart file on rank %10d\n", r);
             }
-            auto t1 = clock::now();
-            t_integrate += std::chrono::duration<double>(t1 - t0).count();
+            const auto t2 = clock::now();
             if (ldpcp_run_lanes(h, 0, (uint32_t)W, lanes.data())) die("ldpcp_run_lanes");
             if (ldpcp_run_msd(h, interval.data(), (uint32_t)nl, msd0.data())) die("ldpcp_run_msd");
             if (cfg.waiting && ldpcp_run_ticks(h, 0, (uint32_t)W, ticks.data())) die("ldpcp_run_ticks");
@@ -476,7 +489,7 @@ int run(const RunnerConfig& cfg)
                 state_count[r][0] += L.state_count[0];
                 state_count[r][1] += L.state_count[1];
             }
-            t_analysis += std::chrono::duration<double>(clock::now() - t1).count();
+            t_analysis += std::chrono::duration<double>(clock::now() - t2).count();
         }
         fprintf(stderr, "   Done %10d/%10d\n", i + 1, cfg.ntrial);
     }
