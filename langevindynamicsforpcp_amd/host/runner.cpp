@@ -363,135 +363,146 @@ int run(const RunnerConfig& cfg)
     std::vector<std::vector<double>> state0_prev(R, std::vector<double>(W, 0.0));
     std::vector<char> image;
 
-    for (int i = 0; i < cfg.ntrial; ++i) {
-        fprintf(stderr, "##################################################################\n");
-        fprintf(stderr, "Trial %10d/%10d\n", i, cfg.ntrial);
-        fprintf(stderr, "##################################################################\n");
-        fprintf(stderr, "   Integrate\n");
-        // every rank's integrator is launched before any is waited for: the ranks
-        // run concurrently on the GPU, as the reference's MPI ranks do
+    // Trials are independent unless UNIFORMINITALSTATES carries each lane's
+    // initial state into the next one, so up to 64 handles (ranks x trials) are
+    // launched before any is waited for; the post analysis then runs in the
+    // reference's order, trial by trial and rank by rank.
+    const int batch = cfg.same_initial_barrier ? 1 : std::max(1, 64 / R);
+    for (int i0 = 0; i0 < cfg.ntrial; i0 += batch) {
+        const int i1 = std::min(cfg.ntrial, i0 + batch);
         const auto t0 = clock::now();
-        std::vector<ldpcp_run*> handles(R, nullptr);
-        std::vector<uint64_t> before(R, 0);
-        for (int r = 0; r < R; ++r) {
-            p.seed = (uint64_t)(uint32_t)seeds[r];
-            ldpcp_run* h = nullptr;
-            if (ldpcp_run_create(&p, (uint32_t)W, (uint32_t)((uint64_t)i * W), &h)) die("ldpcp_run_create");
-            if (cfg.same_initial_barrier && ldpcp_run_set_barrier_default(h, state0_prev[r].data()))
-                die("ldpcp_run_set_barrier_default");
-            uint64_t nsteps_before = 0;  // m_NumberOfStepDonePreviously
-            if (cfg.read_restart) {
-                // PreIntegrate -> ReadRestartFile (Single.h:1346-1351); a failure ends the
-                // run as the runner's integrator-failure check does (:3436-3446)
-                ImageSpec spec;  // what a reference program's file must become for this handle
-                spec.seed = p.seed;
-                spec.traj_offset = (uint32_t)((uint64_t)i * W);
-                spec.barrier_window = p.barrier_window;
-                spec.n_fields = cfg.write_x ? 7 : 1;
-                spec.tick_capacity = p.tick_capacity;
-                bool ok = read_restart_file(cfg, W, r, i, spec, image, &nsteps_before);
-                if (ok && ldpcp_run_resume(h, image.data(), image.size())) {
-                    fprintf(stderr, "Error: %s\n", ldpcp_last_error());
-                    ok = false;
+        std::vector<ldpcp_run*> handles((size_t)(i1 - i0) * R, nullptr);
+        std::vector<uint64_t> before(handles.size(), 0);
+        for (int i = i0; i < i1; ++i)
+            for (int r = 0; r < R; ++r) {
+                p.seed = (uint64_t)(uint32_t)seeds[r];
+                ldpcp_run* h = nullptr;
+                if (ldpcp_run_create(&p, (uint32_t)W, (uint32_t)((uint64_t)i * W), &h)) die("ldpcp_run_create");
+                if (cfg.same_initial_barrier && ldpcp_run_set_barrier_default(h, state0_prev[r].data()))
+                    die("ldpcp_run_set_barrier_default");
+                uint64_t nsteps_before = 0;  // m_NumberOfStepDonePreviously
+                if (cfg.read_restart) {
+                    // PreIntegrate -> ReadRestartFile (Single.h:1346-1351); a failure ends the
+                    // run as the runner's integrator-failure check does (:3436-3446)
+                    ImageSpec spec;  // what a reference program's file must become for this handle
+                    spec.seed = p.seed;
+                    spec.traj_offset = (uint32_t)((uint64_t)i * W);
+                    spec.barrier_window = p.barrier_window;
+                    spec.n_fields = cfg.write_x ? 7 : 1;
+                    spec.tick_capacity = p.tick_capacity;
+                    bool ok = read_restart_file(cfg, W, r, i, spec, image, &nsteps_before);
+                    if (ok && ldpcp_run_resume(h, image.data(), image.size())) {
+                        fprintf(stderr, "Error: %s\n", ldpcp_last_error());
+                        ok = false;
+                    }
+                    if (!ok) {
+                        fprintf(stderr, "Error: can't read restart file on rank %10d\n", r);
+                        fprintf(stderr, "Error: Integrator failure\n");
+                        ldpcp_run_destroy(h);
+                        for (ldpcp_run* q : handles) ldpcp_run_destroy(q);  // waits for their runs
+                        exit(0);
+                    }
                 }
-                if (!ok) {
-                    fprintf(stderr, "Error: can't read restart file on rank %10d\n", r);
-                    fprintf(stderr, "Error: Integrator failure\n");
-                    ldpcp_run_destroy(h);
-                    for (int q = 0; q < r; ++q) ldpcp_run_destroy(handles[q]);  // waits for their runs
-                    exit(0);
-                }
+                if (ldpcp_run_launch(h, cfg.steps_per_launch)) die("ldpcp_run_launch");
+                handles[(size_t)(i - i0) * R + r] = h;
+                before[(size_t)(i - i0) * R + r] = nsteps_before;
             }
-            if (ldpcp_run_launch(h, cfg.steps_per_launch)) die("ldpcp_run_launch");
-            handles[r] = h;
-            before[r] = nsteps_before;
-        }
-        for (int r = 0; r < R; ++r)
-            if (ldpcp_run_wait(handles[r])) die("ldpcp_run_integrate");
+        for (ldpcp_run* h : handles)
+            if (ldpcp_run_wait(h)) die("ldpcp_run_integrate");
         const auto t1 = clock::now();
         t_integrate += std::chrono::duration<double>(t1 - t0).count();
-        for (int r = 0; r < R; ++r) {
-            ldpcp_run* h = handles[r];
-            const uint64_t nsteps_before = before[r];
-            if (cfg.write_restart) {
-                uint64_t bytes = 0;
-                if (ldpcp_run_checkpoint_size(h, &bytes)) die("ldpcp_run_checkpoint_size");
-                image.resize(bytes);
-                if (ldpcp_run_checkpoint(h, image.data(), bytes)) die("ldpcp_run_checkpoint");
-                if (!write_restart_file(cfg, W, r, i, nsteps_before + cfg.nsteps, image))
-                    fprintf(stderr, "Error: can't open restart file on rank %10d\n", r);
-            }
-            const auto t2 = clock::now();
-            if (ldpcp_run_lanes(h, 0, (uint32_t)W, lanes.data())) die("ldpcp_run_lanes");
-            if (ldpcp_run_msd(h, interval.data(), (uint32_t)nl, msd0.data())) die("ldpcp_run_msd");
-            if (cfg.waiting && ldpcp_run_ticks(h, 0, (uint32_t)W, ticks.data())) die("ldpcp_run_ticks");
-            // frames of this run, after a restart's frames when it continues one
-            const uint64_t nf = ldpcp_run_n_frames(h);
-            if (cfg.write_x) frames.resize((size_t)7 * W * nf);
-            if (cfg.write_x)
-                for (int field = 0; field < 7; ++field)
-                    if (ldpcp_run_frames(h, field, 0, (uint32_t)W, frames.data() + (size_t)field * W * nf))
-                        die("ldpcp_run_frames");
-            ldpcp_run_destroy(h);
-            if (r == 0) fprintf(stderr, "   Post Analysis\n");
-            // per-lane post analysis (Single.h:3452-3580)
-            for (int ch = 0; ch < W; ++ch) {
-                const ldpcp_lane& L = lanes[ch];
-                for (size_t t = 0; t < nl; ++t) {
-                    const double v = msd0[(size_t)ch * nl + t];
-                    msd[r][t] += v;
-                    if (isnan(v) || isinf(v)) fprintf(stderr, "Error: on node %10d, ill msd0\n", r);
+        for (int i = i0; i < i1; ++i) {
+            fprintf(stderr, "##################################################################\n");
+            fprintf(stderr, "Trial %10d/%10d\n", i, cfg.ntrial);
+            fprintf(stderr, "##################################################################\n");
+            fprintf(stderr, "   Integrate\n");
+            for (int r = 0; r < R; ++r) {
+                ldpcp_run* h = handles[(size_t)(i - i0) * R + r];
+                const uint64_t nsteps_before = before[(size_t)(i - i0) * R + r];
+                if (cfg.write_restart) {
+                    uint64_t bytes = 0;
+                    if (ldpcp_run_checkpoint_size(h, &bytes)) die("ldpcp_run_checkpoint_size");
+                    image.resize(bytes);
+                    if (ldpcp_run_checkpoint(h, image.data(), bytes)) die("ldpcp_run_checkpoint");
+                    if (!write_restart_file(cfg, W, r, i, nsteps_before + cfg.nsteps, image))
+                        fprintf(stderr, "Error: can't open restart file on rank %10d\n", r);
                 }
-                if (cfg.waiting) {
-                    if (L.wt_ticks <= tick_cap) {
-                        const double* tk = ticks.data() + (size_t)ch * tick_cap;
-                        for (uint64_t t = 1; t < L.wt_ticks; ++t) wt_list[r].push_back((tk[t] - tk[t - 1]) * cfg.timestep);
-                    } else {
-                        wt_exact = false;
+                const auto t2 = clock::now();
+                if (ldpcp_run_lanes(h, 0, (uint32_t)W, lanes.data())) die("ldpcp_run_lanes");
+                if (ldpcp_run_msd(h, interval.data(), (uint32_t)nl, msd0.data())) die("ldpcp_run_msd");
+                if (cfg.waiting && ldpcp_run_ticks(h, 0, (uint32_t)W, ticks.data())) die("ldpcp_run_ticks");
+                // frames of this run, after a restart's frames when it continues one
+                const uint64_t nf = ldpcp_run_n_frames(h);
+                if (cfg.write_x) frames.resize((size_t)7 * W * nf);
+                if (cfg.write_x)
+                    for (int field = 0; field < 7; ++field)
+                        if (ldpcp_run_frames(h, field, 0, (uint32_t)W, frames.data() + (size_t)field * W * nf))
+                            die("ldpcp_run_frames");
+                ldpcp_run_destroy(h);
+                if (r == 0) fprintf(stderr, "   Post Analysis\n");
+                // per-lane post analysis (Single.h:3452-3580)
+                for (int ch = 0; ch < W; ++ch) {
+                    const ldpcp_lane& L = lanes[ch];
+                    for (size_t t = 0; t < nl; ++t) {
+                        const double v = msd0[(size_t)ch * nl + t];
+                        msd[r][t] += v;
+                        if (isnan(v) || isinf(v)) fprintf(stderr, "Error: on node %10d, ill msd0\n", r);
                     }
-                    wt_sum[r] += L.wt_sum;
-                    wt_sum2[r] += L.wt_sum2;
-                    wt_n[r] += (double)L.wt_n;
-                    if (L.wt_n > 0) wt_max[r] = std::max(wt_max[r], L.wt_max);
-                }
-                state0_prev[r][ch] = L.state0_instance;
-                if (cfg.v0_distribution) h_v0[r].add(L.v0_instance);
-                if (cfg.state0_distribution) h_s0[r].add(L.state0_instance);
-                if (cfg.write_x) {
-                    char buf[512];
-                    snprintf(buf, sizeof(buf), "%s_trajectory_%010d_%010d_%010d.log", cfg.output_base.c_str(), i, ch, r);
-                    FILE* f = open_output(buf);
-                    if (f) {
-                        hd.print(f);
-                        auto fr = [&](int field, uint64_t t) { return frames[((size_t)field * W + ch) * nf + t]; };
-                        for (uint64_t t = 0; t < nf; ++t)
-                            fprintf(f, " %23.16f %13.7f %13.7f %13.7f %13.7f %13.7f %13.7f %13.7f\n",
-                                    hd.save_dt * (double)t, fr(0, t), cfg.overdamped ? 0.0 : fr(1, t), fr(2, t),
-                                    fr(6, t), fr(3, t), cfg.overdamped ? 0.0 : fr(4, t), fr(5, t));
-                        if (cfg.waiting) {
-                            fprintf(f, "# event tick\n");
-                            const uint64_t n = std::min<uint64_t>(L.wt_ticks, tick_cap);
-                            for (uint64_t t = 0; t < n; ++t)
-                                fprintf(f, "# %10zd %23.16E\n", (size_t)t, ticks[(size_t)ch * tick_cap + t] * cfg.timestep);
+                    if (cfg.waiting) {
+                        if (L.wt_ticks <= tick_cap) {
+                            const double* tk = ticks.data() + (size_t)ch * tick_cap;
+                            for (uint64_t t = 1; t < L.wt_ticks; ++t)
+                                wt_list[r].push_back((tk[t] - tk[t - 1]) * cfg.timestep);
+                        } else {
+                            wt_exact = false;
                         }
-                        fclose(f);
+                        wt_sum[r] += L.wt_sum;
+                        wt_sum2[r] += L.wt_sum2;
+                        wt_n[r] += (double)L.wt_n;
+                        if (L.wt_n > 0) wt_max[r] = std::max(wt_max[r], L.wt_max);
                     }
+                    state0_prev[r][ch] = L.state0_instance;
+                    if (cfg.v0_distribution) h_v0[r].add(L.v0_instance);
+                    if (cfg.state0_distribution) h_s0[r].add(L.state0_instance);
+                    if (cfg.write_x) {
+                        char buf[512];
+                        snprintf(buf, sizeof(buf), "%s_trajectory_%010d_%010d_%010d.log", cfg.output_base.c_str(), i,
+                                 ch, r);
+                        FILE* f = open_output(buf);
+                        if (f) {
+                            hd.print(f);
+                            auto fr = [&](int field, uint64_t t) { return frames[((size_t)field * W + ch) * nf + t]; };
+                            for (uint64_t t = 0; t < nf; ++t)
+                                fprintf(f, " %23.16f %13.7f %13.7f %13.7f %13.7f %13.7f %13.7f %13.7f\n",
+                                        hd.save_dt * (double)t, fr(0, t), cfg.overdamped ? 0.0 : fr(1, t), fr(2, t),
+                                        fr(6, t), fr(3, t), cfg.overdamped ? 0.0 : fr(4, t), fr(5, t));
+                            if (cfg.waiting) {
+                                fprintf(f, "# event tick\n");
+                                const uint64_t n = std::min<uint64_t>(L.wt_ticks, tick_cap);
+                                for (uint64_t t = 0; t < n; ++t)
+                                    fprintf(f, "# %10zd %23.16E\n", (size_t)t,
+                                            ticks[(size_t)ch * tick_cap + t] * cfg.timestep);
+                            }
+                            fclose(f);
+                        }
+                    }
+                    if (cfg.fpt) {
+                        fpt[r].push_back(L.fpt_n ? L.fpt_sum / (double)L.fpt_n : 0.0);
+                        fpt2[r].push_back(L.fpt2_n ? L.fpt2_sum / (double)L.fpt2_n : 0.0);
+                    }
+                    if (cfg.tpt) {
+                        tpt[r].push_back(L.tpt_n ? L.tpt_sum / (double)L.tpt_n : 0.0);
+                        tpt2[r].push_back(L.tpt2_n ? L.tpt2_sum / (double)L.tpt2_n : 0.0);
+                    }
+                    for (size_t j = 0; j < ncg; ++j)
+                        cg[j][r].push_back(L.cg_n[j] ? L.cg_sum[j] / (double)L.cg_n[j] : 0.0);
+                    state_count[r][0] += L.state_count[0];
+                    state_count[r][1] += L.state_count[1];
                 }
-                if (cfg.fpt) {
-                    fpt[r].push_back(L.fpt_n ? L.fpt_sum / (double)L.fpt_n : 0.0);
-                    fpt2[r].push_back(L.fpt2_n ? L.fpt2_sum / (double)L.fpt2_n : 0.0);
-                }
-                if (cfg.tpt) {
-                    tpt[r].push_back(L.tpt_n ? L.tpt_sum / (double)L.tpt_n : 0.0);
-                    tpt2[r].push_back(L.tpt2_n ? L.tpt2_sum / (double)L.tpt2_n : 0.0);
-                }
-                for (size_t j = 0; j < ncg; ++j) cg[j][r].push_back(L.cg_n[j] ? L.cg_sum[j] / (double)L.cg_n[j] : 0.0);
-                state_count[r][0] += L.state_count[0];
-                state_count[r][1] += L.state_count[1];
+                t_analysis += std::chrono::duration<double>(clock::now() - t2).count();
             }
-            t_analysis += std::chrono::duration<double>(clock::now() - t2).count();
+            fprintf(stderr, "   Done %10d/%10d\n", i + 1, cfg.ntrial);
         }
-        fprintf(stderr, "   Done %10d/%10d\n", i + 1, cfg.ntrial);
     }
     const auto t_post0 = clock::now();
     const std::string& ob = cfg.output_base;
