@@ -232,3 +232,31 @@ def test_reference_restart_files_import_to_the_recorded_state(harness, tmp_path)
                 e = int(win[i, 0, rk["barrier_min_id"] + k - wbase[i]])
                 assert e >> 63 == 1 and (e >> 62) & 1 == rk["barrier_state"][k][i]
                 assert e & ((1 << 62) - 1) == rk["barrier_step"][k][i]
+
+
+def test_host_restart_code_is_clean_under_sanitizers(tmp_path):
+    """The same harness built with AddressSanitizer and UndefinedBehaviorSanitizer
+    (host code only): writing, reading, rejecting a truncated file and, where
+    the reference's recorded files are present, importing one of them."""
+    (tmp_path / "h.cpp").write_text(HARNESS)
+    exe = tmp_path / "h_san"
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined",
+                           "-fno-sanitize-recover=undefined", "-I", os.path.join(REPO, "include"), "-I", HOST,
+                           str(tmp_path / "h.cpp"), os.path.join(HOST, "restart_file.cpp"),
+                           os.path.join(HOST, "input_parser.cpp"), os.path.join(HOST, "text_util.cpp"),
+                           "-o", str(exe)])
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0")
+
+    def go(*args, rc=0):
+        r = subprocess.run([str(exe), *args], cwd=tmp_path, capture_output=True, text=True, env=env)
+        assert r.returncode == rc and "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr
+        return r
+
+    go("write", "0")
+    assert go("read", "0").stdout.split() == ["123456", "1"]
+    path = tmp_path / "rs_00001_00002.restart"
+    path.write_bytes(path.read_bytes()[:-10])
+    go("read", "0", rc=2)
+    if os.path.isdir(QUICK_JOB):
+        go("import", os.path.join(GOLDEN, "quick_test.inp"), os.path.join(QUICK_JOB, "m1.0_t1.00e-05_restart_0"),
+           "2", "4", str(tmp_path / "img"))
