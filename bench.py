@@ -226,6 +226,37 @@ def serving(seconds, budget_gb, rate=20.0, seed=11, fp64_limit_s=60.0):
     return out
 
 
+def run_mode_c1():
+    """The drop-in executable on the 32-trajectory case (tests/golden/c1_32traj.inp:
+    underdamped single barrier 7.2/2.0, X0=-2.9 at rest, seed 1, 4 ranks x 8 lanes,
+    1e7 steps, every statistic on): FP64 reference arithmetic, the ranks'
+    handles running concurrently.  The integrate time is the one the program
+    writes to c1_time.log."""
+    import re
+    import shutil
+    import tempfile
+    exe = os.path.join(REPO, "langevindynamicsforpcp_amd", "bin", "langevinDynamicsSingleBarrier")
+    d = tempfile.mkdtemp(prefix="ldpcp_c1_")
+    try:
+        shutil.copyfile(os.path.join(REPO, "tests", "golden", "c1_32traj.inp"), os.path.join(d, "c1.inp"))
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, "-i", "c1.inp", "--ranks", "4", "--simd-width", "8"], cwd=d, capture_output=True,
+                           text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr[-2000:])
+        log = open(os.path.join(d, "c1_time.log")).read()
+        integ = float(re.search(r"Integrate\s+([0-9.]+) s", log).group(1))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    steps = 32 * 10**7
+    return {"workload": "drop-in executable, single barrier underdamped, 4 ranks x 8 lanes x 1e7 steps, FP64",
+            "integrate_s": integ, "wall_s": wall, "traj_steps_per_s": steps / integ,
+            "reference_published_traj_steps_per_s": 1.12e8,
+            "reference_note": "the reference's quick test on 4 CPU cores (16 trajectories x 3e9 overdamped steps in "
+                              "7.13 min); it publishes no figure for this case"}
+
+
 def cpu_baseline(seconds=12.0):
     """The CPU restatement (oracle, lane mode, same Philox noise and step),
     one process per host core, on a bounded sample of the same workload: the
@@ -363,6 +394,7 @@ def main():
     ap.add_argument("--no-ragged", action="store_true", help="skip the 256 ragged parameter sets")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs")
+    ap.add_argument("--no-run-mode", action="store_true", help="skip the drop-in executable's 32-trajectory case")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -419,6 +451,11 @@ def main():
         rag = ragged()
         print(f"# ragged {rag}", file=sys.stderr, flush=True)
 
+    run_mode = None
+    if not args.no_run_mode:
+        run_mode = run_mode_c1()
+        print(f"# run mode {run_mode}", file=sys.stderr, flush=True)
+
     serve = None
     if not args.no_serve:
         serve = serving(args.serve_seconds, args.serve_budget_gb)
@@ -472,6 +509,8 @@ def main():
         line["double_barrier"] = dbl
     if rag:
         line["ragged"] = rag
+    if run_mode:
+        line["run_mode"] = run_mode
     if serve:
         line["serving"] = serve
     if not args.no_cpu_baseline:

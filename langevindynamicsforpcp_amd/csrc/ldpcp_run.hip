@@ -879,7 +879,15 @@ StepKernel step_kernel_for(const RunConst& c, int src)
 // Handles with at most this many trajectories fill a small part of the GPU, so
 // each step's latency decides their time: they take the buffered noise.
 constexpr uint32_t kBufferedMaxTraj = 8192;
-constexpr uint64_t kBufferedLaunch = 8192;  // steps per launch (and per noise buffer)
+constexpr uint64_t kBufferedLaunch = 8192;       // most steps per launch (and per noise buffer)
+constexpr uint64_t kNoiseBufferBytes = 1ull << 28;  // noise buffer bound: fewer steps per launch above 2048 trajectories
+
+// Steps per launch of a buffered-noise handle of n trajectories.
+uint64_t buffered_steps(uint64_t n)
+{
+    const uint64_t fit = kNoiseBufferBytes / (n * sizeof(uint4));
+    return fit < kBufferedLaunch ? fit : kBufferedLaunch;
+}
 
 }  // namespace
 
@@ -892,7 +900,7 @@ struct ldpcp_run {
     hipStream_t stream;
     void* state_block;
     void* frames_block;   // frames of a resumed run (prior frames + this run's), else null
-    uint4* noise;         // buffered noise, [kBufferedLaunch][n_traj], allocated on first use
+    uint4* noise;         // buffered noise, [buffered_steps(n_traj)][n_traj], for handles of few trajectories
     uint64_t n_frames;
     uint64_t step_base;   // absolute step the next integrate starts after
     int phase;            // 0 fresh, 1 resumed (next integrate continues), 2 resumed run done
@@ -1000,6 +1008,16 @@ extern "C" int ldpcp_run_create(const ldpcp_run_params* p, uint32_t n_traj, uint
         delete h;
         return LDPCP_EDEVICE;
     }
+    // the buffered-noise scratch is allocated here, not at launch: an allocation
+    // between launches could wait for other handles' runs
+    if (n <= kBufferedMaxTraj) e = hipMalloc(&h->noise, buffered_steps(n) * n * sizeof(uint4));
+    if (e != hipSuccess) {
+        set_error("ldpcp_run_create: hipMalloc of the noise buffer failed: %s", hipGetErrorString(e));
+        (void)hipStreamDestroy(h->stream);
+        (void)hipFree(h->state_block);
+        delete h;
+        return LDPCP_ENOMEM;
+    }
     e = hipMemsetAsync(h->P.bdefault, 0, bytes_bdef, h->stream);
     if (e != hipSuccess) {
         set_error("ldpcp_run_create: hipMemsetAsync failed: %s", hipGetErrorString(e));
@@ -1095,8 +1113,7 @@ extern "C" int ldpcp_run_launch(ldpcp_run* h, uint64_t steps_per_launch)
     const int src = h->c.replay ? kNoiseReplay : buffered ? kNoiseBuffered : kNoisePhilox;
     const StepKernel step = step_kernel_for(h->c, src);
     if (buffered) {
-        if (steps_per_launch > kBufferedLaunch) steps_per_launch = kBufferedLaunch;
-        if (!h->noise) LDPCP_HIP_TRY(hipMalloc(&h->noise, kBufferedLaunch * n * sizeof(uint4)));
+        if (steps_per_launch > buffered_steps(n)) steps_per_launch = buffered_steps(n);
     }
     const dim3 sblock(buffered ? 64 : 256), sgrid((unsigned)((n + sblock.x - 1) / sblock.x));
     for (uint64_t s0 = h->step_base; s0 < end; s0 += steps_per_launch) {
