@@ -1064,10 +1064,17 @@ extern "C" int ldpcp_run_set_replay(ldpcp_run* h, const double* gauss, uint64_t 
     double* c = (double*)b;
     b += n * catch_cap * sizeof(double);
     int8_t* bi = (int8_t*)b;
-    LDPCP_HIP_TRY(hipMemcpy(g, gauss, n_gauss * n * sizeof(double), hipMemcpyHostToDevice));
-    if (n_flip) LDPCP_HIP_TRY(hipMemcpy(f, flip, n_flip * n * sizeof(double), hipMemcpyHostToDevice));
-    if (catch_cap) LDPCP_HIP_TRY(hipMemcpy(c, catchup, n * catch_cap * sizeof(double), hipMemcpyHostToDevice));
-    if (bn) LDPCP_HIP_TRY(hipMemcpy(bi, binit, 2 * n * (uint64_t)bn, hipMemcpyHostToDevice));
+    hipError_t e = hipMemcpy(g, gauss, n_gauss * n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n_flip) e = hipMemcpy(f, flip, n_flip * n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && catch_cap)
+        e = hipMemcpy(c, catchup, n * catch_cap * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && bn) e = hipMemcpy(bi, binit, 2 * n * (uint64_t)bn, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(block);
+        set_error("ldpcp_run_set_replay: copy to the device failed: %s", hipGetErrorString(e));
+        return LDPCP_EDEVICE;
+    }
+    if (h->c.replay) (void)hipFree((void*)h->P.g);  // the streams of an earlier call
     h->P.g = g;
     h->P.f = f;
     h->P.c = c;

@@ -148,3 +148,32 @@ def test_gpu_buffered_noise_equals_inline_noise(name):
         np.testing.assert_array_equal(a.frames(field), b.frames(field)[:small])
     a.close()
     b.close()
+
+
+def test_gpu_replay_streams_can_be_replaced(oracle):
+    """A second ldpcp_run_set_replay on a handle replaces the first call's
+    streams (and frees them): the run follows the second set."""
+    p = make("single_under", save_all=1)
+    W = 8
+    lanes, frames, s = run_rank_recorded(oracle, p, W, seed=23)
+    ng, nf = min(s["n_gauss"]), min(s["n_flip"])
+    le = to_engine(p, W)
+    le.barrier_window = 1024
+    le._create()
+    from langevindynamicsforpcp_amd import _native as N
+    lib = N.load()
+    g, f = np.ascontiguousarray(s["gauss"][:, :ng].T), np.ascontiguousarray(s["flip"][:, :nf].T)
+    c, b = np.ascontiguousarray(s["catch"]), np.ascontiguousarray(s["binit"], np.int8)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    junk = np.zeros_like(g)
+    for gauss in (junk, g):
+        N.check(lib.ldpcp_run_set_replay(le._h, dp(gauss), g.shape[0], dp(f), f.shape[0], dp(c), c.shape[1],
+                                         b.ctypes.data_as(C.POINTER(C.c_int8)), s["bmin"], b.shape[2]),
+                "ldpcp_run_set_replay")
+    N.check(lib.ldpcp_run_integrate(le._h, 4096), "ldpcp_run_integrate")
+    got = (N.LaneResult * W)()
+    N.check(lib.ldpcp_run_lanes(le._h, 0, W, got), "ldpcp_run_lanes")
+    for i in range(W):
+        for fld in EXACT + COUNTS:
+            assert getattr(got[i], fld) == getattr(lanes[i], fld), (i, fld)
+    le.close()
