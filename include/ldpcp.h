@@ -52,7 +52,22 @@ typedef struct ldpcp_run_params {
     uint64_t seed;                 /* Philox key (free-running noise) */
     int32_t barrier_window;        /* barrier entries kept per trajectory and barrier array */
     uint64_t tick_capacity;        /* barrier-crossing ticks kept per trajectory (0 = none) */
+    uint64_t event_capacity;       /* entries kept per event list per trajectory (0 = none): the
+                                      FPT / FPT2 / TPT / TPT2 / FPTCG lists and the FPT1 barrier
+                                      logs the reference keeps for its restart files */
 } ldpcp_run_params;
+
+/* Event lists kept per trajectory when event_capacity > 0, in this order
+ * (TrajectoryData, langevinDynamicsSingleBarrier.h:160-235): FPT1, FPT2,
+ * TPT1, TPT2, the FPT1 logs of the barrier crossed (m_InstantBarrierList) and
+ * of the barrier left (m_InitialBarrierList), then FPTCG for each size. */
+#define LDPCP_LIST_FPT 0
+#define LDPCP_LIST_FPT2 1
+#define LDPCP_LIST_TPT 2
+#define LDPCP_LIST_TPT2 3
+#define LDPCP_LIST_INSTANT 4
+#define LDPCP_LIST_INITIAL 5
+#define LDPCP_LIST_CG0 6
 
 /* Per-trajectory results: the values the runner reads through
  * GetFirstPassageTimeAverage{,2}, GetTransitionPathTimeAverage{,2},
@@ -76,6 +91,11 @@ typedef struct ldpcp_lane {
     uint64_t absorbed_step, tpt_start_step, steps_done;
     uint64_t n_gauss, n_flip, n_catch;
     int32_t error;
+    /* FPT1 barrier log state (m_InitialBarrierIndex / m_InitialBarrierState,
+     * Single.h:2364-2387): the barriers either side of the well the last FPT1
+     * started from */
+    int32_t fpt_init_index[2];
+    double fpt_init_state[2];
 } ldpcp_lane;
 
 typedef struct ldpcp_run ldpcp_run;
@@ -149,11 +169,14 @@ int ldpcp_run_set_barrier_default(ldpcp_run* h, const double* state);
  *                                                      bits 0-61 step of the last hand-over)
  *   double         frames[n_fields][n_frames][n_traj] (x; with WRITEX v, f, pe, ke, state, barrier)
  *   double         ticks[n_traj][tick_capacity]
- * Event lists are held as running sums and counts (ldpcp_lane), and the noise
+ *   double         events: for each trajectory, for each of the n_lists lists,
+ *                  its first min(count, event_capacity) entries (counts in ldpcp_lane:
+ *                  fpt_n for FPT1 and both barrier logs, fpt2_n, tpt_n, tpt2_n, cg_n[j])
+ * Event lists are also held as running sums and counts (ldpcp_lane), and the noise
  * position is the step count itself (the Philox counter), so a resumed run
  * continues bit for bit: NSTEPS = a + b in one run equals a run of a steps,
  * a checkpoint, and a resumed run of b steps. */
-#define LDPCP_CHECKPOINT_MAGIC 0x314B43504350444Cull /* "LDPCPCK1" */
+#define LDPCP_CHECKPOINT_MAGIC 0x324B43504350444Cull /* "LDPCPCK2" */
 typedef struct ldpcp_checkpoint_header {
     uint64_t magic;
     uint32_t header_bytes, lane_bytes;  /* sizeof of the two structs, as written */
@@ -167,7 +190,25 @@ typedef struct ldpcp_checkpoint_header {
     uint64_t tick_capacity;
     double timestep;
     uint64_t bytes;                     /* size of the whole image */
+    uint64_t event_capacity;
+    uint32_t n_lists;                   /* event lists per trajectory (6 + FPTCG sizes, or 0) */
+    uint32_t events_complete;           /* 1: the lists hold every event since step 0 */
 } ldpcp_checkpoint_header;
+
+/* Number of events list `list` holds for a trajectory (entries kept:
+ * the smaller of this and event_capacity). */
+static inline uint64_t ldpcp_list_count(const ldpcp_lane* l, int list)
+{
+    switch (list) {
+    case LDPCP_LIST_FPT:
+    case LDPCP_LIST_INSTANT:
+    case LDPCP_LIST_INITIAL: return l->fpt_n;
+    case LDPCP_LIST_FPT2: return l->fpt2_n;
+    case LDPCP_LIST_TPT: return l->tpt_n;
+    case LDPCP_LIST_TPT2: return l->tpt2_n;
+    default: return l->cg_n[list - LDPCP_LIST_CG0];
+    }
+}
 
 /* Size of the image ldpcp_run_checkpoint writes for this handle now. */
 int ldpcp_run_checkpoint_size(ldpcp_run* h, uint64_t* bytes);
@@ -179,6 +220,15 @@ int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes);
  * the image's; ldpcp_run_n_frames then counts both.  Such a handle integrates
  * once. */
 int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes);
+
+/* Replaces the barrier table the reference's restart file lists
+ * (FiniteInifiniteIntegerTickArray1D::Write, DataLocator.h:163-169): the
+ * stored state (0/1) and last hand-over step of barriers lo .. hi of `array`
+ * for every trajectory, out as [n_traj][hi - lo + 1].  Barriers a trajectory
+ * has not touched are created with their step-0 state, as the reference's
+ * table lookups create them. */
+int ldpcp_run_barrier_table(ldpcp_run* h, int32_t array, int32_t lo, int32_t hi, int32_t* states,
+                            uint64_t* steps);
 
 /* Replaces SAFE_DELETE(leOMP) (langevinDynamicsSingleBarrier.h:3972). */
 int ldpcp_run_destroy(ldpcp_run* h);

@@ -39,6 +39,7 @@ class RunParams(C.Structure):
         ("seed", C.c_uint64),
         ("barrier_window", C.c_int32),
         ("tick_capacity", C.c_uint64),
+        ("event_capacity", C.c_uint64),
     ]
 
 
@@ -63,6 +64,7 @@ class LaneResult(C.Structure):
         ("absorbed_step", C.c_uint64), ("tpt_start_step", C.c_uint64), ("steps_done", C.c_uint64),
         ("n_gauss", C.c_uint64), ("n_flip", C.c_uint64), ("n_catch", C.c_uint64),
         ("error", C.c_int32),
+        ("fpt_init_index", C.c_int32 * 2), ("fpt_init_state", C.c_double * 2),
     ]
 
 
@@ -110,7 +112,7 @@ class ServerConfig(C.Structure):
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
-CHECKPOINT_MAGIC = 0x314B43504350444C  # "LDPCPCK1"
+CHECKPOINT_MAGIC = 0x324B43504350444C  # "LDPCPCK2"
 
 
 class CheckpointHeader(C.Structure):
@@ -119,7 +121,8 @@ class CheckpointHeader(C.Structure):
                 ("overdamped", C.c_int32), ("barrier_window", C.c_int32), ("n_fields", C.c_int32),
                 ("seed", C.c_uint64), ("steps_done", C.c_uint64), ("n_frames", C.c_uint64),
                 ("save_freq", C.c_uint64), ("tick_capacity", C.c_uint64), ("timestep", C.c_double),
-                ("bytes", C.c_uint64)]
+                ("bytes", C.c_uint64), ("event_capacity", C.c_uint64), ("n_lists", C.c_uint32),
+                ("events_complete", C.c_uint32)]
 
 # (name, restype, argtypes) for every entry point of include/ldpcp.h
 _P = C.c_void_p
@@ -140,6 +143,8 @@ _SIGNATURES = [
     ("ldpcp_run_checkpoint_size", C.c_int, [_P, C.POINTER(C.c_uint64)]),
     ("ldpcp_run_checkpoint", C.c_int, [_P, C.c_void_p, C.c_uint64]),
     ("ldpcp_run_resume", C.c_int, [_P, C.c_void_p, C.c_uint64]),
+    ("ldpcp_run_barrier_table", C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_uint64)]),
     ("ldpcp_run_destroy", C.c_int, [_P]),
     ("ldpcp_sweep_create", C.c_int, [C.POINTER(SweepSet), C.c_int32, C.c_int32, C.POINTER(_P)]),
     ("ldpcp_sweep_launch", C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),

@@ -40,6 +40,8 @@ struct RunConst {
     uint64_t save_freq, nsteps_eff, frames_cap;
     uint32_t key0, key1, n_traj, traj_offset;
     uint64_t n_gauss, n_flip, catch_cap, init_gauss, init_flip, tick_cap;
+    uint64_t ev_cap;   // entries kept per event list per trajectory
+    int32_t n_lists, pad_;
     int32_t bmin, bn;
 };
 
@@ -56,6 +58,7 @@ struct RunPtrs {
     const int8_t* binit; // replay barrier states [array][traj][bn]
     double* ticks;       // barrier ticks [traj][tick_cap]
     double* bdefault;    // state of barriers created with UNIFORMINITALSTATES [traj]
+    double* events;      // event lists [traj][list][ev_cap]
 };
 
 // The run's branch flags, separate from RunConst so that a kernel can make them
@@ -162,6 +165,8 @@ void make_const(const ldpcp_run_params& p, RunConst& c)
     c.init_gauss = (uint64_t)(p.randomize_v ? 1 : 0) + (uint64_t)(p.overdamped ? 1 : 0);
     c.init_flip = p.randomize_state ? 1 : 0;
     c.tick_cap = p.tick_capacity;
+    c.ev_cap = p.event_capacity;
+    c.n_lists = p.event_capacity ? LDPCP_LIST_CG0 + (p.b_fptcg ? p.n_cg : 0) : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -387,7 +392,44 @@ __device__ __forceinline__ void arm_indices(int idx[2], const Lane& s)
     }
 }
 
-__device__ __forceinline__ void stats_init(const RunConst& c, ldpcp_lane* o, const Lane& s)
+// Stored state of barrier `id` of the single barrier's array, created (its
+// state at step 0) if this trajectory has not touched it yet.
+__device__ __forceinline__ int stored_state(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t, int id,
+                                            int& err)
+{
+    uint64_t* e = win_entry(c, P, t, 0, id, err);
+    if (!e) return 0;
+    return (win_load(c, f, P, t, 0, id, e, err) & kState) ? 1 : 0;
+}
+
+// FPT1 arming (Single.h:2364-2387; at step 0 :2184-2212): the barriers either
+// side of the well the particle sits in, the one under the particle with its
+// current state, the other with its stored state.
+__device__ __forceinline__ void fpt1_arm(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
+                                         ldpcp_lane* o, const Lane& s, int& err)
+{
+    const int pidx = (int)s.periodic;
+    if (s.well == s.periodic) {
+        o->fpt_init_state[0] = s.state;
+        o->fpt_init_index[0] = pidx;
+        o->fpt_init_state[1] = (double)stored_state(c, f, P, t, pidx + 1, err);
+        o->fpt_init_index[1] = pidx + 1;
+    } else {
+        o->fpt_init_state[0] = (double)stored_state(c, f, P, t, pidx - 1, err);
+        o->fpt_init_index[0] = pidx - 1;
+        o->fpt_init_state[1] = s.state;
+        o->fpt_init_index[1] = pidx;
+    }
+}
+
+// Appends v as entry n of list `list` when the lists are kept.
+__device__ __forceinline__ void push_event(const RunConst& c, double* ev, int list, uint64_t n, double v)
+{
+    if (n < c.ev_cap) ev[(uint64_t)list * c.ev_cap + n] = v;
+}
+
+__device__ __forceinline__ void stats_init(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
+                                           ldpcp_lane* o, const Lane& s, int& err)
 {
     if (c.b_wait) {
         o->wt_tick_prev = 0.0;
@@ -396,6 +438,7 @@ __device__ __forceinline__ void stats_init(const RunConst& c, ldpcp_lane* o, con
     if (c.b_fpt) {
         o->fpt_last_tick = 0.0;
         o->fpt_to_update = c.centre ? 0 : 1;
+        if (f.model == 0 && c.centre) fpt1_arm(c, f, P, t, o, s, err);
         o->fpt2_last_tick = 0.0;
         if (c.centre) {
             o->fpt2_to_update = 0;
@@ -427,15 +470,16 @@ __device__ __forceinline__ void stats_init(const RunConst& c, ldpcp_lane* o, con
 // 2525, 2576): a scalar statement the reference's compiler contracts into one
 // fused multiply-add; the recorded event lists of the quick test match only
 // this form.
-__device__ __forceinline__ double ev(const RunConst& c, uint64_t iStep, double last)
+__device__ __forceinline__ double duration(const RunConst& c, uint64_t iStep, double last)
 {
     return fma(c.dt, (double)iStep, -last);
 }
 
-__device__ __forceinline__ void stats_event(const RunConst& c, const Flags& f, ldpcp_lane* o, const Lane& s,
-                                            uint64_t iStep, double* ticks)
+__device__ __forceinline__ void stats_event(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t traj,
+                                            ldpcp_lane* o, const Lane& s, uint64_t iStep, double* ticks, int& err)
 {
     const double t = c.dt * (double)iStep;
+    double* const ev = c.ev_cap ? P.events + (uint64_t)traj * c.n_lists * c.ev_cap : nullptr;
     const bool single = (f.model == 0);
     if (c.b_wait && single) {
         if (s.ent_well) {
@@ -461,10 +505,20 @@ __device__ __forceinline__ void stats_event(const RunConst& c, const Flags& f, l
             if (o->fpt_to_update && s.ent_period) {
                 o->fpt_to_update = 0;
                 o->fpt_last_tick = t;
+                fpt1_arm(c, f, P, traj, o, s, err);
             }
             if (!o->fpt_to_update && s.ent_well) {
                 o->fpt_to_update = 1;
-                o->fpt_sum += ev(c, iStep, o->fpt_last_tick);
+                const double d = duration(c, iStep, o->fpt_last_tick);
+                o->fpt_sum += d;
+                if (ev) {
+                    // the barrier logs (Single.h:2398-2404)
+                    push_event(c, ev, LDPCP_LIST_FPT, o->fpt_n, d);
+                    push_event(c, ev, LDPCP_LIST_INITIAL, o->fpt_n,
+                               s.periodic == (double)o->fpt_init_index[0] ? o->fpt_init_state[0]
+                                                                           : o->fpt_init_state[1]);
+                    push_event(c, ev, LDPCP_LIST_INSTANT, o->fpt_n, s.state);
+                }
                 o->fpt_n++;
             }
         }
@@ -478,7 +532,9 @@ __device__ __forceinline__ void stats_event(const RunConst& c, const Flags& f, l
         }
         if (s.ent_period && !o->fpt2_to_update && ((double)o->fpt2_idx[0] != s.periodic) &&
             ((double)o->fpt2_idx[1] != s.periodic)) {
-            o->fpt2_sum += ev(c, iStep, o->fpt2_last_tick);
+            const double d = duration(c, iStep, o->fpt2_last_tick);
+            o->fpt2_sum += d;
+            if (ev) push_event(c, ev, LDPCP_LIST_FPT2, o->fpt2_n, d);
             o->fpt2_n++;
             o->fpt2_last_tick = t;
             int idx[2];
@@ -494,14 +550,18 @@ __device__ __forceinline__ void stats_event(const RunConst& c, const Flags& f, l
         }
         if (!o->tpt_to_update && s.ent_well) {
             o->tpt_to_update = 1;
-            o->tpt_sum += ev(c, iStep, o->tpt_last_tick);
+            const double d = duration(c, iStep, o->tpt_last_tick);
+            o->tpt_sum += d;
+            if (ev) push_event(c, ev, LDPCP_LIST_TPT, o->tpt_n, d);
             o->tpt_n++;
         }
         if (s.ent_period) {
             if (o->tpt2_to_update) {
                 o->tpt2_to_update = 0;
             } else if (s.well != (double)o->tpt2_initial_well) {
-                o->tpt2_sum += ev(c, iStep, o->tpt2_last_tick);
+                const double d = duration(c, iStep, o->tpt2_last_tick);
+                o->tpt2_sum += d;
+                if (ev) push_event(c, ev, LDPCP_LIST_TPT2, o->tpt2_n, d);
                 o->tpt2_n++;
             }
             o->tpt2_last_tick = t;
@@ -516,7 +576,9 @@ __device__ __forceinline__ void stats_event(const RunConst& c, const Flags& f, l
                 o->cg_last_tick[j] = t;
                 o->cg_initial_well[j] = (int)s.well;
             } else if (s.well != (double)o->cg_initial_well[j]) {
-                o->cg_sum[j] += ev(c, iStep, o->cg_last_tick[j]);
+                const double d = duration(c, iStep, o->cg_last_tick[j]);
+                o->cg_sum[j] += d;
+                if (ev) push_event(c, ev, LDPCP_LIST_CG0 + j, o->cg_n[j], d);
                 o->cg_n[j]++;
                 o->cg_last_tick[j] = t;
                 o->cg_initial_well[j] = (int)s.well;
@@ -622,7 +684,7 @@ __global__ void __launch_bounds__(256) run_init_kernel(RunConst c, RunPtrs P)
     }
     o->v0_instance = s.v;
     lane_energy_force(s, c, f);
-    stats_init(c, o, s);
+    stats_init(c, f, P, t, o, s, err);
     if (f.over) {
         const double g = f.replay ? P.g[ng * c.n_traj + t] : gr;
         ng++;
@@ -799,7 +861,10 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         lane_param(s, c, f);
         lane_energy_force(s, c, f);
         if (!f.over) s.v = s.v + (c.minv * s.F) * c.halfdt;
-        if (s.ent_period || s.ent_well) stats_event(c, f, o, s, iStep, P.ticks + (uint64_t)t * c.tick_cap);
+        if (s.ent_period || s.ent_well) {
+            stats_event(c, f, P, t, o, s, iStep, P.ticks + (uint64_t)t * c.tick_cap, err);
+            if (err) break;
+        }
         if (--to_save == 0) {
             record(c, f, P, t, s, frame++);
             to_save = c.save_freq;
@@ -854,6 +919,34 @@ __global__ void __launch_bounds__(256) run_noise_kernel(RunConst c, uint64_t s0,
     noise[k * c.n_traj + t] = make_uint4((uint32_t)bits, (uint32_t)(bits >> 32), second ? w.w[3] : w.w[2], 0u);
 }
 
+// The barrier table as the reference's restart file lists it (DataLocator.h:163-169):
+// the stored state and hand-over step of barriers lo .. hi of one array, per
+// trajectory.  A barrier the trajectory has not touched is created with its
+// step-0 state, as the reference's table lookup creates it (DataLocator.h:100-188).
+__global__ void __launch_bounds__(256) run_table_kernel(RunConst c, RunPtrs P, int array, int lo, int n_ids,
+                                                        int32_t* __restrict__ states, uint64_t* __restrict__ steps)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = (int)blockIdx.y;
+    if (t >= c.n_traj || k >= n_ids) return;
+    const Flags f = run_flags(c);
+    const int id = lo + k;
+    int err = 0;
+    uint64_t* e = win_entry(c, P, t, array, id, err);
+    int st;
+    uint64_t step = 0;
+    if (e) {
+        const uint64_t v = win_load(c, f, P, t, array, id, e, err);
+        st = (v & kState) ? 1 : 0;
+        step = v & kStepMask;
+    } else {
+        err = 0;  // beyond the trajectory's window: its state at creation, not stored
+        st = barrier_init_state(c, f, P, t, array, id, err);
+    }
+    states[(uint64_t)t * n_ids + k] = st;
+    steps[(uint64_t)t * n_ids + k] = step;
+}
+
 using StepKernel = void (*)(RunConst, RunPtrs, uint64_t, uint64_t, const uint4*);
 
 template <int MODEL, int OVER, int SRC>
@@ -901,6 +994,7 @@ struct ldpcp_run {
     void* state_block;
     void* frames_block;   // frames of a resumed run (prior frames + this run's), else null
     uint4* noise;         // buffered noise, [buffered_steps(n_traj)][n_traj], for handles of few trajectories
+    bool events_complete; // the event lists hold every event since step 0
     uint64_t n_frames;
     uint64_t step_base;   // absolute step the next integrate starts after
     int phase;            // 0 fresh, 1 resumed (next integrate continues), 2 resumed run done
@@ -999,6 +1093,7 @@ extern "C" int ldpcp_run_create(const ldpcp_run_params* p, uint32_t n_traj, uint
     h->P.ticks = (double*)b;
     b += bytes_ticks;
     h->P.bdefault = (double*)b;
+    h->P.events = nullptr;
     h->P.g = h->P.f = h->P.c = nullptr;
     h->P.binit = nullptr;
     e = hipStreamCreate(&h->stream);
@@ -1008,11 +1103,14 @@ extern "C" int ldpcp_run_create(const ldpcp_run_params* p, uint32_t n_traj, uint
         delete h;
         return LDPCP_EDEVICE;
     }
-    // the buffered-noise scratch is allocated here, not at launch: an allocation
-    // between launches could wait for other handles' runs
+    // the buffered-noise scratch and the event lists are allocated here, not at
+    // launch: an allocation between launches could wait for other handles' runs
     if (n <= kBufferedMaxTraj) e = hipMalloc(&h->noise, buffered_steps(n) * n * sizeof(uint4));
+    if (e == hipSuccess && h->c.n_lists)
+        e = hipMalloc(&h->P.events, n * (uint64_t)h->c.n_lists * h->c.ev_cap * sizeof(double));
     if (e != hipSuccess) {
-        set_error("ldpcp_run_create: hipMalloc of the noise buffer failed: %s", hipGetErrorString(e));
+        set_error("ldpcp_run_create: hipMalloc of the noise buffer or event lists failed: %s", hipGetErrorString(e));
+        if (h->noise) (void)hipFree(h->noise);
         (void)hipStreamDestroy(h->stream);
         (void)hipFree(h->state_block);
         delete h;
@@ -1106,10 +1204,15 @@ extern "C" int ldpcp_run_launch(ldpcp_run* h, uint64_t steps_per_launch)
     const uint64_t n = h->c.n_traj;
     const dim3 block(256), grid((unsigned)((n + 255) / 256));
     if (h->phase == 0) {
+        h->events_complete = h->c.n_lists > 0;
         LDPCP_HIP_TRY(hipMemsetAsync(h->P.lanes, 0, n * sizeof(ldpcp_lane), h->stream));
         LDPCP_HIP_TRY(hipMemsetAsync(h->P.win, 0, n * 2 * (uint64_t)h->c.wb * sizeof(uint64_t), h->stream));
         const uint64_t fields = h->params.save_all ? kFields : 1;
         LDPCP_HIP_TRY(hipMemsetAsync(h->P.frames, 0, fields * h->c.frames_cap * n * sizeof(double), h->stream));
+        // unused tick slots are part of a checkpoint image: keep them zero, as a resumed
+        // handle has them, so that images of equal states are equal bytes
+        if (h->c.tick_cap)
+            LDPCP_HIP_TRY(hipMemsetAsync(h->P.ticks, 0, n * h->c.tick_cap * sizeof(double), h->stream));
         hipLaunchKernelGGL(run_init_kernel, grid, block, 0, h->stream, h->c, h->P);
         LDPCP_HIP_TRY(hipGetLastError());
     }
@@ -1276,10 +1379,23 @@ extern "C" int ldpcp_run_msd(ldpcp_run* h, const uint64_t* lags, uint32_t nlags,
 namespace {
 
 struct CkptLayout {
-    uint64_t lanes, wbase, win, frames, ticks, total;
+    uint64_t lanes, wbase, win, frames, ticks, events, total;
 };
 
-CkptLayout ckpt_layout(uint64_t n, uint64_t wb, uint64_t fields, uint64_t n_frames, uint64_t tick_cap)
+// entries the image holds for the event lists: min(count, capacity) per list
+uint64_t packed_events(const ldpcp_lane* lanes, uint64_t n, uint32_t n_lists, uint64_t cap)
+{
+    uint64_t k = 0;
+    for (uint64_t t = 0; t < n; ++t)
+        for (uint32_t l = 0; l < n_lists; ++l) {
+            const uint64_t c = ldpcp_list_count(lanes + t, (int)l);
+            k += c < cap ? c : cap;
+        }
+    return k;
+}
+
+CkptLayout ckpt_layout(uint64_t n, uint64_t wb, uint64_t fields, uint64_t n_frames, uint64_t tick_cap,
+                       uint64_t n_events)
 {
     CkptLayout L;
     uint64_t o = sizeof(ldpcp_checkpoint_header);
@@ -1293,6 +1409,8 @@ CkptLayout ckpt_layout(uint64_t n, uint64_t wb, uint64_t fields, uint64_t n_fram
     o += fields * n_frames * n * sizeof(double);
     L.ticks = o;
     o += n * tick_cap * sizeof(double);
+    L.events = o;
+    o += n_events * sizeof(double);
     L.total = o;
     return L;
 }
@@ -1302,10 +1420,10 @@ uint64_t run_fields(const ldpcp_run* h) { return h->params.save_all ? kFields : 
 // absolute step of the state a checkpoint would hold
 uint64_t run_steps_done(const ldpcp_run* h) { return h->phase == 1 ? h->step_base : h->step_base + h->c.nsteps_eff; }
 
-CkptLayout run_layout(const ldpcp_run* h)
+CkptLayout run_layout(const ldpcp_run* h, uint64_t n_events)
 {
     return ckpt_layout(h->c.n_traj, (uint64_t)h->c.wb, run_fields(h), run_steps_done(h) / h->c.save_freq + 1,
-                       h->c.tick_cap);
+                       h->c.tick_cap, n_events);
 }
 
 }  // namespace
@@ -1335,7 +1453,14 @@ extern "C" int ldpcp_run_checkpoint_size(ldpcp_run* h, uint64_t* bytes)
         set_error("ldpcp_run_checkpoint_size: null argument");
         return LDPCP_EINVAL;
     }
-    *bytes = run_layout(h).total;
+    uint64_t n_events = 0;
+    if (h->c.n_lists) {
+        std::vector<ldpcp_lane> lanes(h->c.n_traj);
+        const int rc = ldpcp_run_lanes(h, 0, h->c.n_traj, lanes.data());
+        if (rc) return rc;
+        n_events = packed_events(lanes.data(), h->c.n_traj, (uint32_t)h->c.n_lists, h->c.ev_cap);
+    }
+    *bytes = run_layout(h, n_events).total;
     return LDPCP_OK;
 }
 
@@ -1354,7 +1479,12 @@ extern "C" int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes)
         return LDPCP_EINVAL;
     }
     const uint64_t n = h->c.n_traj, wb = (uint64_t)h->c.wb, fields = run_fields(h);
-    const CkptLayout L = run_layout(h);
+    std::vector<ldpcp_lane> lv(n);
+    int rc = ldpcp_run_lanes(h, 0, (uint32_t)n, lv.data());
+    if (rc) return rc;
+    const uint32_t n_lists = (uint32_t)h->c.n_lists;
+    const uint64_t ecap = h->c.ev_cap;
+    const CkptLayout L = run_layout(h, packed_events(lv.data(), n, n_lists, ecap));
     if (bytes < L.total) {
         set_error("ldpcp_run_checkpoint: buffer of %llu bytes, need %llu", (unsigned long long)bytes,
                   (unsigned long long)L.total);
@@ -1362,8 +1492,7 @@ extern "C" int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes)
     }
     char* b = (char*)buf;
     ldpcp_lane* lanes = (ldpcp_lane*)(b + L.lanes);
-    int rc = ldpcp_run_lanes(h, 0, (uint32_t)n, lanes);
-    if (rc) return rc;
+    memcpy(lanes, lv.data(), n * sizeof(ldpcp_lane));
     for (uint64_t t = 0; t < n; ++t)
         if (lanes[t].error) {
             set_error("ldpcp_run_checkpoint: trajectory %llu stopped with error %d; its state is not resumable",
@@ -1388,7 +1517,25 @@ extern "C" int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes)
     hd.tick_capacity = h->c.tick_cap;
     hd.timestep = h->params.timestep;
     hd.bytes = L.total;
+    hd.event_capacity = ecap;
+    hd.n_lists = n_lists;
+    bool complete = h->events_complete && n_lists;
+    for (uint64_t t = 0; complete && t < n; ++t)
+        for (uint32_t l = 0; l < n_lists; ++l) complete = complete && ldpcp_list_count(lanes + t, (int)l) <= ecap;
+    hd.events_complete = complete ? 1u : 0u;
     memcpy(b, &hd, sizeof(hd));
+    if (n_lists) {
+        // the lists, packed: each trajectory's kept entries, list after list
+        std::vector<double> all(n * n_lists * ecap);
+        LDPCP_HIP_TRY(hipMemcpy(all.data(), h->P.events, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+        double* dst = (double*)(b + L.events);
+        for (uint64_t t = 0; t < n; ++t)
+            for (uint32_t l = 0; l < n_lists; ++l) {
+                const uint64_t c = ldpcp_list_count(lanes + t, (int)l), keep = c < ecap ? c : ecap;
+                memcpy(dst, &all[(t * n_lists + l) * ecap], keep * sizeof(double));
+                dst += keep;
+            }
+    }
     LDPCP_HIP_TRY(hipMemcpy(b + L.wbase, h->P.wbase, n * sizeof(int32_t), hipMemcpyDeviceToHost));
     LDPCP_HIP_TRY(hipMemcpy(b + L.win, h->P.win, n * 2 * wb * sizeof(uint64_t), hipMemcpyDeviceToHost));
     // the frames recorded so far: the first n_frames rows of each field's plane
@@ -1439,7 +1586,17 @@ extern "C" int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes)
             set_error("ldpcp_run_resume: inconsistent %s", k.what);
             return LDPCP_EINVAL;
         }
-    const CkptLayout L = ckpt_layout(n, wb, fields, hd.n_frames, hd.tick_capacity);
+    if (hd.n_lists && h->c.n_lists && hd.n_lists != (uint32_t)h->c.n_lists) {
+        set_error("ldpcp_run_resume: inconsistent event lists (FPTCG sizes)");
+        return LDPCP_EINVAL;
+    }
+    const ldpcp_lane* ilanes = (const ldpcp_lane*)((const char*)buf + sizeof(ldpcp_checkpoint_header));
+    if (bytes < sizeof(ldpcp_checkpoint_header) + n * sizeof(ldpcp_lane)) {
+        set_error("ldpcp_run_resume: truncated image");
+        return LDPCP_EINVAL;
+    }
+    const CkptLayout L = ckpt_layout(n, wb, fields, hd.n_frames, hd.tick_capacity,
+                                     packed_events(ilanes, n, hd.n_lists, hd.event_capacity));
     if (hd.bytes != L.total || bytes < L.total) {
         set_error("ldpcp_run_resume: image of %llu bytes, its header describes %llu", (unsigned long long)bytes,
                   (unsigned long long)L.total);
@@ -1505,9 +1662,59 @@ extern "C" int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes)
             memcpy(&tk[t * h->c.tick_cap], b + L.ticks + t * hd.tick_capacity * sizeof(double), keep * sizeof(double));
         LDPCP_HIP_TRY(hipMemcpy(h->P.ticks, tk.data(), tk.size() * sizeof(double), hipMemcpyHostToDevice));
     }
+    // event lists: the image's kept entries, as many as this handle keeps
+    const uint64_t ecap = h->c.ev_cap;
+    const uint32_t n_lists = (uint32_t)h->c.n_lists;
+    bool complete = n_lists && hd.events_complete && hd.n_lists == n_lists;
+    if (n_lists) {
+        std::vector<double> all(n * n_lists * ecap, 0.0);
+        const double* src = (const double*)(b + L.events);
+        for (uint64_t t = 0; t < n; ++t)
+            for (uint32_t l = 0; l < hd.n_lists; ++l) {
+                const uint64_t c = ldpcp_list_count(lanes + t, (int)l);
+                const uint64_t have = c < hd.event_capacity ? c : hd.event_capacity;
+                const uint64_t keep = have < ecap ? have : ecap;
+                complete = complete && c <= ecap;
+                memcpy(&all[(t * n_lists + l) * ecap], src, keep * sizeof(double));
+                src += have;
+            }
+        LDPCP_HIP_TRY(hipMemcpy(h->P.events, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    h->events_complete = complete;
     h->step_base = hd.steps_done;
     h->phase = 1;
     h->integrated = false;
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_barrier_table(ldpcp_run* h, int32_t array, int32_t lo, int32_t hi, int32_t* states,
+                                       uint64_t* steps)
+{
+    if (h && h->pending) {  // results of a launched run: wait for it first
+        const int rc = ldpcp_run_wait(h);
+        if (rc) return rc;
+    }
+    if (!h || !states || !steps || hi < lo || array < 0 || array > 1 || (uint64_t)hi - lo + 1 > 65535) {
+        set_error("ldpcp_run_barrier_table: bad handle, array, range or output");
+        return LDPCP_EINVAL;
+    }
+    const uint64_t n = h->c.n_traj, ids = (uint64_t)(hi - lo) + 1;
+    void* block = nullptr;
+    LDPCP_HIP_TRY(hipMalloc(&block, n * ids * (sizeof(int32_t) + sizeof(uint64_t))));
+    uint64_t* d_steps = (uint64_t*)block;
+    int32_t* d_states = (int32_t*)(d_steps + n * ids);
+    const dim3 grid((unsigned)((n + 255) / 256), (unsigned)ids);
+    hipLaunchKernelGGL(run_table_kernel, grid, dim3(256), 0, h->stream, h->c, h->P, (int)array, (int)lo, (int)ids,
+                       d_states, d_steps);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(states, d_states, n * ids * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(steps, d_steps, n * ids * sizeof(uint64_t), hipMemcpyDeviceToHost);
+    (void)hipFree(block);
+    if (e != hipSuccess) {
+        set_error("ldpcp_run_barrier_table: %s", hipGetErrorString(e));
+        return LDPCP_EDEVICE;
+    }
     return LDPCP_OK;
 }
 
@@ -1518,6 +1725,7 @@ extern "C" int ldpcp_run_destroy(ldpcp_run* h)
     if (h->c.replay) (void)hipFree((void*)h->P.g);
     if (h->frames_block) (void)hipFree(h->frames_block);
     if (h->noise) (void)hipFree(h->noise);
+    if (h->P.events) (void)hipFree(h->P.events);
     (void)hipFree(h->state_block);
     (void)hipStreamDestroy(h->stream);
     delete h;

@@ -105,7 +105,9 @@ struct RefLane {
     double tick_prev = 0.0;
     int passed = 0;
     std::vector<double> ticks;
-    std::vector<double> fpt, fpt2, tpt, tpt2;
+    std::vector<double> fpt, fpt2, tpt, tpt2, inst, init;
+    double fpt_init_state[2] = {0.0, 0.0};
+    int fpt_init_index[2] = {0, 0};
     double fpt_last = 0.0, fpt2_last = 0.0, tpt_last = 0.0, tpt2_last = 0.0;
     int fpt_up = 0, fpt2_up = 0, tpt_up = 0, tpt2_up = 0, tpt2_well = 0, fpt2_idx[2] = {0, 0};
     std::vector<int> cg_size;
@@ -149,14 +151,13 @@ bool read_ref_lane(Reader& r, const RunnerConfig& c, bool wt, bool fpt, RefLane&
         if (!r.vec(L.ticks)) return false;
     }
     if (fpt) {
-        std::vector<double> skip;
-        if (!r.vec(L.fpt) || !r.vec(skip) || !r.vec(skip)) return false;  // barrier logs feed no output
+        if (!r.vec(L.fpt) || !r.vec(L.inst) || !r.vec(L.init)) return false;
         L.fpt_last = r.f64();
         L.fpt_up = r.i32();
-        r.f64();
-        r.f64();
-        r.i32();
-        r.i32();
+        L.fpt_init_state[0] = r.f64();
+        L.fpt_init_state[1] = r.f64();
+        L.fpt_init_index[0] = r.i32();
+        L.fpt_init_index[1] = r.i32();
         if (!r.vec(L.fpt2)) return false;
         L.fpt2_last = r.f64();
         L.fpt2_up = r.i32();
@@ -226,7 +227,18 @@ bool import_reference_body(Reader& r, FILE* f, const RunnerConfig& c, int W, con
     if (c.overdamped)
         for (auto& v : r_prev) v = r.f64();
     if (r.i32() != kRestartEnd || r.eof()) return fail(f, "restart file corrupted");
-    if (fgetc(f) != EOF) return fail(f, "restart file corrupted");
+    // a file this program wrote carries its exact state after the end marker
+    const int next = fgetc(f);
+    if (next != EOF) {
+        ungetc(next, f);
+        const uint64_t magic = r.u64(), size = r.u64();
+        if (r.eof() || magic != LDPCP_CHECKPOINT_MAGIC || size > (1ull << 40))
+            return fail(f, "restart file corrupted");
+        image.resize(size);
+        if (!r.bytes(image.data(), size) || fgetc(f) != EOF) return fail(f, "restart file corrupted");
+        fclose(f);
+        return true;
+    }
 
     // checks the resumed run relies on
     const uint64_t n_frames = lanes[0].traj[0].size();
@@ -255,7 +267,30 @@ bool import_reference_body(Reader& r, FILE* f, const RunnerConfig& c, int W, con
     const uint64_t o_win = o_wbase + ((n * sizeof(int32_t) + 7) & ~7ull);
     const uint64_t o_frames = o_win + n * 2 * wb * sizeof(uint64_t);
     const uint64_t o_ticks = o_frames + (uint64_t)nfields * n_frames * n * sizeof(double);
-    const uint64_t total = o_ticks + n * cap * sizeof(double);
+    const uint64_t o_events = o_ticks + n * cap * sizeof(double);
+    // the event lists in the library's order (LDPCP_LIST_*), at most event_capacity entries each
+    const uint32_t n_lists = spec.n_lists;
+    const uint64_t ecap = spec.event_capacity;
+    auto list_of = [&](const RefLane& R, uint32_t l) -> const std::vector<double>& {
+        switch (l) {
+        case LDPCP_LIST_FPT: return R.fpt;
+        case LDPCP_LIST_FPT2: return R.fpt2;
+        case LDPCP_LIST_TPT: return R.tpt;
+        case LDPCP_LIST_TPT2: return R.tpt2;
+        case LDPCP_LIST_INSTANT: return R.inst;
+        case LDPCP_LIST_INITIAL: return R.init;
+        default: return R.cg[l - LDPCP_LIST_CG0];
+        }
+    };
+    uint64_t n_events = 0;
+    bool complete = n_lists > 0;
+    for (int i = 0; i < W; ++i)
+        for (uint32_t l = 0; l < n_lists; ++l) {
+            const uint64_t len = list_of(lanes[i], l).size();
+            n_events += len < ecap ? len : ecap;
+            complete = complete && len <= ecap;
+        }
+    const uint64_t total = o_events + n_events * sizeof(double);
     image.assign(total, 0);
     char* b = image.data();
 
@@ -277,6 +312,9 @@ bool import_reference_body(Reader& r, FILE* f, const RunnerConfig& c, int W, con
     hd.tick_capacity = cap;
     hd.timestep = c.timestep;
     hd.bytes = total;
+    hd.event_capacity = ecap;
+    hd.n_lists = n_lists;
+    hd.events_complete = complete ? 1u : 0u;
     memcpy(b, &hd, sizeof(hd));
 
     constexpr uint64_t kValid = 1ull << 63, kState = 1ull << 62, kStepMask = kState - 1;
@@ -339,6 +377,10 @@ bool import_reference_body(Reader& r, FILE* f, const RunnerConfig& c, int W, con
             L.wt_last = R.ticks[k];
         }
         L.steps_done = nsteps_total;
+        L.fpt_init_state[0] = R.fpt_init_state[0];
+        L.fpt_init_state[1] = R.fpt_init_state[1];
+        L.fpt_init_index[0] = R.fpt_init_index[0];
+        L.fpt_init_index[1] = R.fpt_init_index[1];
         memcpy(b + o_lanes + i * sizeof(ldpcp_lane), &L, sizeof(L));
         // barrier table -> the lane's window, centred on its current barrier
         const int32_t base = (int32_t)R.periodic - spec.barrier_window / 2;
@@ -356,6 +398,14 @@ bool import_reference_body(Reader& r, FILE* f, const RunnerConfig& c, int W, con
         const uint64_t keep = R.ticks.size() < cap ? R.ticks.size() : cap;
         if (keep) memcpy(b + o_ticks + (uint64_t)i * cap * sizeof(double), R.ticks.data(), keep * sizeof(double));
     }
+    char* ev = b + o_events;
+    for (int i = 0; i < W; ++i)
+        for (uint32_t l = 0; l < n_lists; ++l) {
+            const std::vector<double>& v = list_of(lanes[i], l);
+            const uint64_t keep = v.size() < ecap ? v.size() : ecap;
+            if (keep) memcpy(ev, v.data(), keep * sizeof(double));
+            ev += keep * sizeof(double);
+        }
     fclose(f);
     return true;
 }
@@ -369,18 +419,15 @@ std::string restart_file_name(const std::string& base, int rank, int trial)
     return base + buf;
 }
 
-bool write_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, uint64_t nsteps_total,
-                        const std::vector<char>& image)
+namespace {
+
+// The settings header and the three noise positions (Single.h:1054-1101;
+// Double.h:1208-1255).  The noise positions are those of this program: the
+// Philox counter of the step noise is the step itself.
+void write_header(Writer& w, const RunnerConfig& c, int signature, int simd_width, int rank, uint64_t nsteps_total)
 {
-    const std::string name = restart_file_name(c.output_restart, rank, trial);
-    FILE* f = fopen(name.c_str(), "wb");
-    if (!f) {
-        fprintf(stderr, "Error: can't open restart file (%s)\n", name.c_str());
-        return false;
-    }
-    Writer w(f);
     const SaveFlags sf = save_flags(c);
-    w.i32(kRestartSignature);
+    w.i32(signature);
     w.i32(simd_width);
     w.f64(c.x0);
     w.f64(c.v0);
@@ -423,24 +470,227 @@ bool write_restart_file(const RunnerConfig& c, int simd_width, int rank, int tri
     w.i32(sf.state);
     w.i32(sf.barrier);
     w.i32(rank);
-    // noise positions: the Philox counter of the step noise is the step itself;
-    // the scalar catch-up positions live per trajectory in the image
     w.put<int64_t>((int64_t)nsteps_total);
     w.u64(0);
     w.put<int64_t>((int64_t)nsteps_total);
     w.u64(0);
     w.put<int64_t>(0);
     w.u64(0);
-    w.i32(kBodyStart);
-    w.u64(image.size());
-    w.bytes(image.data(), image.size());
-    w.i32(kRestartEnd);
+}
+
+bool finish(FILE* f, const Writer& w, const std::string& name)
+{
     const bool ok = w.ok();
     if (fclose(f) != 0 || !ok) {
         fprintf(stderr, "Error: can't open restart file (%s)\n", name.c_str());
         return false;
     }
     return true;
+}
+
+}  // namespace
+
+bool write_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, uint64_t nsteps_total,
+                        const std::vector<char>& image)
+{
+    const std::string name = restart_file_name(c.output_restart, rank, trial);
+    FILE* f = fopen(name.c_str(), "wb");
+    if (!f) {
+        fprintf(stderr, "Error: can't open restart file (%s)\n", name.c_str());
+        return false;
+    }
+    Writer w(f);
+    write_header(w, c, kRestartSignature, simd_width, rank, nsteps_total);
+    w.i32(kBodyStart);
+    w.u64(image.size());
+    w.bytes(image.data(), image.size());
+    w.i32(kRestartEnd);
+    return finish(f, w, name);
+}
+
+bool reference_layout_possible(const RunnerConfig& c, const std::vector<char>& image)
+{
+    if (c.model != 0 || image.size() < sizeof(ldpcp_checkpoint_header)) return false;
+    ldpcp_checkpoint_header hd;
+    memcpy(&hd, image.data(), sizeof(hd));
+    if (!hd.events_complete) return false;
+    const ldpcp_lane* lanes = (const ldpcp_lane*)(image.data() + sizeof(hd));
+    for (uint32_t t = 0; t < hd.n_traj; ++t)
+        if (c.waiting && lanes[t].wt_ticks > hd.tick_capacity) return false;
+    return true;
+}
+
+void barrier_span(const std::vector<char>& image, int* lo, int* hi)
+{
+    ldpcp_checkpoint_header hd;
+    memcpy(&hd, image.data(), sizeof(hd));
+    const uint64_t n = hd.n_traj, wb = (uint64_t)hd.barrier_window;
+    const char* b = image.data() + sizeof(hd) + n * sizeof(ldpcp_lane);
+    const char* win = b + ((n * sizeof(int32_t) + 7) & ~7ull);
+    for (uint64_t t = 0; t < n; ++t) {
+        int32_t base;
+        memcpy(&base, b + t * sizeof(int32_t), sizeof(base));
+        for (uint64_t k = 0; k < wb; ++k) {
+            uint64_t e;
+            memcpy(&e, win + ((t * 2) * wb + k) * sizeof(uint64_t), sizeof(e));
+            if (!(e >> 63)) continue;  // never created
+            const int id = base + (int)k;
+            if (id < *lo) *lo = id;
+            if (id > *hi) *hi = id;
+        }
+    }
+}
+
+bool write_reference_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, uint64_t nsteps_total,
+                                  const std::vector<char>& image, int table_lo, int table_n,
+                                  const std::vector<int32_t>& states, const std::vector<uint64_t>& steps)
+{
+    ldpcp_checkpoint_header hd;
+    memcpy(&hd, image.data(), sizeof(hd));
+    const uint64_t n = hd.n_traj, wb = (uint64_t)hd.barrier_window, nf = hd.n_frames, cap = hd.tick_capacity;
+    const char* b = image.data();
+    const ldpcp_lane* lanes = (const ldpcp_lane*)(b + sizeof(hd));
+    const uint64_t o_wbase = sizeof(hd) + n * sizeof(ldpcp_lane);
+    const uint64_t o_frames = o_wbase + ((n * sizeof(int32_t) + 7) & ~7ull) + n * 2 * wb * sizeof(uint64_t);
+    const uint64_t o_ticks = o_frames + (uint64_t)hd.n_fields * nf * n * sizeof(double);
+    const uint64_t o_events = o_ticks + n * cap * sizeof(double);
+    auto frame = [&](int field, uint64_t fr, uint64_t t) {
+        double v;
+        memcpy(&v, b + o_frames + (((uint64_t)field * nf + fr) * n + t) * sizeof(double), sizeof(v));
+        return v;
+    };
+    // packed event lists: trajectory after trajectory, list after list
+    std::vector<uint64_t> list_at(n * hd.n_lists);
+    {
+        uint64_t k = 0;
+        for (uint64_t t = 0; t < n; ++t)
+            for (uint32_t l = 0; l < hd.n_lists; ++l) {
+                list_at[t * hd.n_lists + l] = k;
+                const uint64_t cnt = ldpcp_list_count(lanes + t, (int)l);
+                k += cnt < hd.event_capacity ? cnt : hd.event_capacity;
+            }
+    }
+    const std::string name = restart_file_name(c.output_restart, rank, trial);
+    FILE* f = fopen(name.c_str(), "wb");
+    if (!f) {
+        fprintf(stderr, "Error: can't open restart file (%s)\n", name.c_str());
+        return false;
+    }
+    Writer w(f);
+    const SaveFlags sf = save_flags(c);
+    write_header(w, c, kReferenceSignature, simd_width, rank, nsteps_total);
+    // TrajectoryData::WriteRestartFile (Single.h:433-572)
+    w.i32(kBodyStart);
+    w.i32(c.waiting ? 1 : 0);
+    w.i32(c.fpt ? 1 : 0);
+    auto list = [&](uint64_t t, int l) {
+        const uint64_t cnt = ldpcp_list_count(lanes + t, l);
+        w.u64(cnt);
+        w.bytes(b + o_events + list_at[t * hd.n_lists + l] * sizeof(double), cnt * sizeof(double));
+    };
+    // saved-frame lists x, v, f, pe, ke, eta, state, barrier -> this image's fields
+    const int field_of[8] = {0, sf.v ? 1 : -1, sf.f ? 2 : -1, sf.e ? 3 : -1, sf.e ? 4 : -1, -1, sf.state ? 5 : -1,
+                             sf.barrier ? 6 : -1};
+    for (uint64_t t = 0; t < n; ++t) {
+        const ldpcp_lane& L = lanes[t];
+        const int st = L.state == 1.0 ? 1 : 0;
+        const double height = c.height[0][st], slope = c.height[0][st] / c.half_width[0];
+        w.f64(L.x);
+        w.f64(L.v);
+        w.f64(L.f);
+        w.f64(L.e);
+        w.f64(0.0);  // eta: the programs have no eta noise on
+        w.f64(L.barrier);
+        for (int k = 0; k < 8; ++k) {
+            const int fld = field_of[k];
+            if (fld < 0 || fld >= hd.n_fields) {
+                w.u64(0);
+                continue;
+            }
+            w.u64(nf);
+            for (uint64_t fr = 0; fr < nf; ++fr) w.f64(frame(fld, fr, t));
+        }
+        w.f64(L.state);
+        w.f64(L.state);  // previous state: overwritten by the next step before any use
+        w.f64(slope);
+        w.f64(0.0 - slope);
+        w.f64(height);
+        w.f64(L.state_count[0]);
+        w.f64(L.state_count[1]);
+        w.f64(L.well);
+        w.f64(L.periodic);
+        w.f64(L.well);  // previous indices: overwritten by the next step before any use
+        w.f64(L.periodic);
+        w.i32(0);
+        w.i32(0);
+        w.f64(L.v0_instance);
+        w.f64(0.0);
+        w.f64(L.state0_instance);
+        w.f64(L.well);
+        w.f64(L.periodic);
+        w.f64(L.well);
+        w.f64(L.periodic);
+        w.i32(0);
+        w.i32(0);
+        if (c.waiting) {
+            w.f64(L.wt_tick_prev);
+            w.i32(L.wt_passed);
+            w.u64(L.wt_ticks);
+            w.bytes(b + o_ticks + t * cap * sizeof(double), L.wt_ticks * sizeof(double));
+        }
+        if (c.fpt) {
+            list(t, LDPCP_LIST_FPT);
+            list(t, LDPCP_LIST_INSTANT);
+            list(t, LDPCP_LIST_INITIAL);
+            w.f64(L.fpt_last_tick);
+            w.i32(L.fpt_to_update);
+            w.f64(L.fpt_init_state[0]);
+            w.f64(L.fpt_init_state[1]);
+            w.i32(L.fpt_init_index[0]);
+            w.i32(L.fpt_init_index[1]);
+            list(t, LDPCP_LIST_FPT2);
+            w.f64(L.fpt2_last_tick);
+            w.i32(L.fpt2_to_update);
+            w.i32(L.fpt2_idx[0]);
+            w.i32(L.fpt2_idx[1]);
+        }
+        if (c.tpt) {
+            list(t, LDPCP_LIST_TPT);
+            w.f64(L.tpt_last_tick);
+            w.i32(L.tpt_to_update);
+            list(t, LDPCP_LIST_TPT2);
+            w.f64(L.tpt2_last_tick);
+            w.i32(L.tpt2_to_update);
+            w.i32(L.tpt2_initial_well);
+        }
+        if (c.fptcg) {
+            w.u64(c.cg_size.size());
+            for (int v : c.cg_size) w.i32(v);
+            for (size_t j = 0; j < c.cg_size.size(); ++j) {
+                list(t, LDPCP_LIST_CG0 + (int)j);
+                w.f64(L.cg_last_tick[j]);
+                w.i32(L.cg_to_update[j]);
+                w.i32(L.cg_initial_well[j]);
+            }
+        }
+    }
+    // the barrier table (DataLocator.h:163-169), then R_prev (:2776-2785)
+    w.u64((uint64_t)table_n);
+    for (int k = 0; k < table_n; ++k)
+        for (uint64_t t = 0; t < n; ++t) {
+            w.i32(states[t * table_n + k]);
+            w.u64(steps[t * table_n + k]);
+        }
+    w.i32(table_lo);
+    if (c.overdamped)
+        for (uint64_t t = 0; t < n; ++t) w.f64(lanes[t].r_prev);
+    w.i32(kRestartEnd);
+    // this program's exact state after the reference's end marker, which the
+    // reference's reader stops at
+    w.u64(LDPCP_CHECKPOINT_MAGIC);
+    w.u64(image.size());
+    w.bytes(image.data(), image.size());
+    return finish(f, w, name);
 }
 
 bool read_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, const ImageSpec& spec,

@@ -5,12 +5,11 @@
 //
 // The file keeps the reference's name, its settings header in the same order
 // and encoding (int 4 B, size_t / long long 8 B, double 8 B, little-endian) and
-// its read-side consistency checks with the same messages.  The body after the
-// header is the library's checkpoint image (ldpcp_checkpoint_header + device
-// state, include/ldpcp.h) in place of the reference's per-lane event lists and
-// MKL bank positions, so the signature differs from the reference's 2345 and
-// the reference program refuses these files at its first check.  This program
-// reads both kinds (read_restart_file).
+// its read-side consistency checks with the same messages.  Two bodies follow
+// the header: the reference's own (signature 2345, write_reference_restart_file),
+// with the library's checkpoint image after the end marker, or the image alone
+// (signature kRestartSignature, write_restart_file), which the reference program
+// refuses at its first check.  This program reads both (read_restart_file).
 #pragma once
 #include <stdint.h>
 
@@ -39,7 +38,28 @@ struct ImageSpec {
     int32_t barrier_window = 0;
     int32_t n_fields = 1;
     uint64_t tick_capacity = 0;
+    uint64_t event_capacity = 0;
+    uint32_t n_lists = 0;
 };
+
+// Whether the image can be written in the reference program's own layout:
+// single barrier, event lists complete (every event since step 0 kept), all
+// waiting-time ticks kept.
+bool reference_layout_possible(const RunnerConfig& c, const std::vector<char>& image);
+
+// Widens [*lo, *hi] to every barrier id of array 0 a trajectory of the image
+// holds in its window.
+void barrier_span(const std::vector<char>& image, int* lo, int* hi);
+
+// WriteRestartFile in the reference program's own layout (signature 2345,
+// TrajectoryData::WriteRestartFile, langevinDynamicsSingleBarrier.h:385-576, the
+// barrier table of ids table_lo .. table_lo + table_n - 1 from
+// ldpcp_run_barrier_table, the overdamped R_prev), readable by the reference
+// program, followed after its end marker by this program's checkpoint image,
+// which read_restart_file prefers (exact continuation).
+bool write_reference_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, uint64_t nsteps_total,
+                                  const std::vector<char>& image, int table_lo, int table_n,
+                                  const std::vector<int32_t>& states, const std::vector<uint64_t>& steps);
 
 // ReadRestartFile: false (after the reference's message on stderr) when the
 // file is missing or any setting differs from c.  On success image holds the

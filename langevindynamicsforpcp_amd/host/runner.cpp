@@ -352,6 +352,11 @@ int run(const RunnerConfig& cfg)
     p.save_all = cfg.write_x;
     p.barrier_window = 4096;
     p.tick_capacity = tick_cap;
+    // restart files in the reference's layout need the raw event lists: a
+    // trajectory keeps up to 256 MB / (W x lists) doubles per list (1M at most)
+    const uint32_t n_lists = LDPCP_LIST_CG0 + (cfg.fptcg ? (uint32_t)ncg : 0u);
+    p.event_capacity =
+        cfg.write_restart ? std::min<uint64_t>(1u << 20, (256ull << 20) / ((uint64_t)W * n_lists * sizeof(double))) : 0;
 
     std::vector<ldpcp_lane> lanes(W);
     std::vector<double> msd0((size_t)W * nl);
@@ -390,6 +395,8 @@ int run(const RunnerConfig& cfg)
                     spec.barrier_window = p.barrier_window;
                     spec.n_fields = cfg.write_x ? 7 : 1;
                     spec.tick_capacity = p.tick_capacity;
+                    spec.event_capacity = p.event_capacity;
+                    spec.n_lists = p.event_capacity ? n_lists : 0;
                     bool ok = read_restart_file(cfg, W, r, i, spec, image, &nsteps_before);
                     if (ok && ldpcp_run_resume(h, image.data(), image.size())) {
                         fprintf(stderr, "Error: %s\n", ldpcp_last_error());
@@ -424,8 +431,23 @@ int run(const RunnerConfig& cfg)
                     if (ldpcp_run_checkpoint_size(h, &bytes)) die("ldpcp_run_checkpoint_size");
                     image.resize(bytes);
                     if (ldpcp_run_checkpoint(h, image.data(), bytes)) die("ldpcp_run_checkpoint");
-                    if (!write_restart_file(cfg, W, r, i, nsteps_before + cfg.nsteps, image))
-                        fprintf(stderr, "Error: can't open restart file on rank %10d\n", r);
+                    bool ok;
+                    if (reference_layout_possible(cfg, image)) {
+                        // the barrier table spans every barrier a trajectory touched and the
+                        // -100 .. 100 the reference creates at step 0 (Single.h:1898-1903)
+                        int lo = -100, hi = 100;
+                        barrier_span(image, &lo, &hi);
+                        const int nb = hi - lo + 1;
+                        std::vector<int32_t> st((size_t)W * nb);
+                        std::vector<uint64_t> sp((size_t)W * nb);
+                        if (ldpcp_run_barrier_table(h, 0, lo, hi, st.data(), sp.data()))
+                            die("ldpcp_run_barrier_table");
+                        ok = write_reference_restart_file(cfg, W, r, i, nsteps_before + cfg.nsteps, image, lo, nb, st,
+                                                          sp);
+                    } else {
+                        ok = write_restart_file(cfg, W, r, i, nsteps_before + cfg.nsteps, image);
+                    }
+                    if (!ok) fprintf(stderr, "Error: can't open restart file on rank %10d\n", r);
                 }
                 const auto t2 = clock::now();
                 if (ldpcp_run_lanes(h, 0, (uint32_t)W, lanes.data())) die("ldpcp_run_lanes");

@@ -51,6 +51,7 @@ class LangevinEquation:
         self.m_SaveAll = False
         self.barrier_window = 1024
         self.tick_capacity = 0
+        self.event_capacity = 0
         self.steps_per_launch = 1 << 20
         self._h = None
         self._seed = 0
@@ -94,6 +95,7 @@ class LangevinEquation:
         p.seed = self._seed & 0xFFFFFFFFFFFFFFFF
         p.barrier_window = self.barrier_window
         p.tick_capacity = self.tick_capacity
+        p.event_capacity = self.event_capacity
         return p
 
     def Initialize(self, seed):
@@ -243,3 +245,38 @@ class LangevinEquation:
 
     def GetStateCount(self, index, state):
         return self.lanes()[index].state_count[state]
+
+
+def read_checkpoint(image):
+    """The parts of a checkpoint image (ldpcp_run_checkpoint; layout in
+    include/ldpcp.h) as arrays: header, lanes, window base and entries,
+    frames [field][frame][trajectory], ticks [trajectory][capacity] and the
+    event lists, events[trajectory][list] (the kept entries)."""
+    buf = bytes(image)
+    hd = N.CheckpointHeader.from_buffer_copy(buf)
+    n, wb, nf, cap = hd.n_traj, hd.barrier_window, hd.n_frames, hd.tick_capacity
+    o = C.sizeof(N.CheckpointHeader)
+    lanes = (N.LaneResult * n).from_buffer_copy(buf[o:o + n * C.sizeof(N.LaneResult)])
+    o += n * C.sizeof(N.LaneResult)
+    wbase = np.frombuffer(buf, np.int32, n, o)
+    o += (n * 4 + 7) // 8 * 8
+    win = np.frombuffer(buf, np.uint64, n * 2 * wb, o).reshape(n, 2, wb)
+    o += n * 2 * wb * 8
+    frames = np.frombuffer(buf, np.float64, hd.n_fields * nf * n, o).reshape(hd.n_fields, nf, n)
+    o += hd.n_fields * nf * n * 8
+    ticks = np.frombuffer(buf, np.float64, n * cap, o).reshape(n, cap)
+    o += n * cap * 8
+    counts = {0: "fpt_n", 1: "fpt2_n", 2: "tpt_n", 3: "tpt2_n", 4: "fpt_n", 5: "fpt_n"}
+    events = []
+    for t in range(n):
+        row = []
+        for k in range(hd.n_lists):
+            c = getattr(lanes[t], counts[k]) if k in counts else lanes[t].cg_n[k - 6]
+            keep = min(c, hd.event_capacity)
+            row.append(np.frombuffer(buf, np.float64, keep, o).copy())
+            o += keep * 8
+        events.append(row)
+    if o != len(buf) or o != hd.bytes:
+        raise ValueError(f"checkpoint image of {len(buf)} bytes, parts end at {o}")
+    return {"header": hd, "lanes": lanes, "wbase": wbase, "window": win, "frames": frames, "ticks": ticks,
+            "events": events}

@@ -229,7 +229,7 @@ static void lane_indices(lstate* s, const consts* c, int model, int init)
 /* look-ups the single-barrier FPT code makes, which create barriers (and so   */
 /* draw uniforms) in rank mode; lane mode passes NULL.                          */
 /* ------------------------------------------------------------------------- */
-typedef void (*touch_fn)(void* ctx, int lane, int id);
+typedef int (*touch_fn)(void* ctx, int lane, int id);  /* returns the barrier's stored state */
 
 static void cap_put(double* base, uint64_t cap, uint64_t n, double v)
 {
@@ -259,6 +259,30 @@ static void fpt_arm_indices(int idx[2], const lstate* s, touch_fn touch, void* t
     }
 }
 
+/* FPT1 arming (Single.h:2364-2387, at step 0 :2184-2212): the barriers on
+ * either side of the well the particle sits in, the one under the particle
+ * with its current state, the other with its stored state (looked up, and so
+ * created, in the order the reference does). */
+static void fpt1_arm(ldo_lane* o, const lstate* s, touch_fn touch, void* tctx, int lane)
+{
+    int pidx = (int)s->periodic;
+    if (s->well == s->periodic) {
+        if (touch) touch(tctx, lane, pidx);
+        int st = touch ? touch(tctx, lane, pidx + 1) : 0;
+        o->fpt_init_state[0] = s->state;
+        o->fpt_init_index[0] = pidx;
+        o->fpt_init_state[1] = (double)st;
+        o->fpt_init_index[1] = pidx + 1;
+    } else {
+        int st = touch ? touch(tctx, lane, pidx - 1) : 0;
+        if (touch) touch(tctx, lane, pidx);
+        o->fpt_init_state[0] = (double)st;
+        o->fpt_init_index[0] = pidx - 1;
+        o->fpt_init_state[1] = s->state;
+        o->fpt_init_index[1] = pidx;
+    }
+}
+
 static void stats_init(const ldo_params* p, ldo_lane* o, const lstate* s, touch_fn touch, void* tctx, int lane)
 {
     double m = fmod(fmod(p->x0, p->periodic_length) + p->periodic_length, p->periodic_length);
@@ -270,11 +294,13 @@ static void stats_init(const ldo_params* p, ldo_lane* o, const lstate* s, touch_
     if (p->b_fpt) {
         int idx[2];
         o->fpt_last_tick = 0.0;
+        (void)idx;
         if (centre) {
             o->fpt_to_update = 0;
-            if (p->model == 0) fpt_arm_indices(idx, s, touch, tctx, lane);
+            if (p->model == 0) fpt1_arm(o, s, touch, tctx, lane);
         } else {
             o->fpt_to_update = 1;
+            o->fpt_init_state[0] = 0.0;
         }
         o->fpt2_last_tick = 0.0;
         if (centre) {
@@ -339,15 +365,18 @@ static void stats_update(const ldo_params* p, ldo_lane* o, const lstate* s, uint
     if (p->b_fpt) {
         if (single) {
             if (o->fpt_to_update && s->ent_period) {
-                int idx[2];
                 o->fpt_to_update = 0;
                 o->fpt_last_tick = t;
-                fpt_arm_indices(idx, s, touch, tctx, lane);
+                fpt1_arm(o, s, touch, tctx, lane);
             }
             if (!o->fpt_to_update && s->ent_well) {
                 o->fpt_to_update = 1;
                 double v = ev(p, iStep, o->fpt_last_tick);
                 cap_put(CAPPTR(fpt, cr), cr ? cr->cap->cap : 0, o->fpt_n, v);
+                /* barrier logs (Single.h:2398-2404) */
+                cap_put(CAPPTR(init, cr), cr ? cr->cap->cap : 0, o->fpt_n,
+                        s->periodic == (double)o->fpt_init_index[0] ? o->fpt_init_state[0] : o->fpt_init_state[1]);
+                cap_put(CAPPTR(inst, cr), cr ? cr->cap->cap : 0, o->fpt_n, s->state);
                 o->fpt_sum += v;
                 o->fpt_n++;
             }
@@ -580,11 +609,12 @@ static int barrier_slot(ldo_rank_state* s, barrier_table* t, int id)
     return id - t->min_id;
 }
 
-static void rank_touch(void* ctx, int lane, int id)
+static int rank_touch(void* ctx, int lane, int id)
 {
     ldo_rank_state* s = (ldo_rank_state*)ctx;
-    (void)lane;
-    barrier_slot(s, &s->bt[s->cur_array], id);
+    barrier_table* t = &s->bt[s->cur_array];
+    int slot = barrier_slot(s, t, id);
+    return t->state[(uint64_t)slot * s->W + lane];
 }
 
 ldo_rank_state* ldo_rank_create(const ldo_params* p, int W, ldo_fill_fn fill, void* ctx, uint64_t bank_size)
@@ -1002,6 +1032,23 @@ static void lane_store_entry(lane_store* t, lane_noise* n, const consts* c, cons
     *slot = k;
 }
 
+/* the statistics' barrier lookups in lane mode: the lane's own store */
+typedef struct lane_touch_ctx {
+    lane_store* st;
+    lane_noise* n;
+    const consts* c;
+    const ldo_params* p;
+} lane_touch_ctx;
+
+static int lane_touch(void* ctx, int lane, int id)
+{
+    lane_touch_ctx* tc = (lane_touch_ctx*)ctx;
+    int k;
+    (void)lane;
+    lane_store_entry(&tc->st[0], tc->n, tc->c, tc->p, 0, id, &k);
+    return tc->st[0].state[k];
+}
+
 static int lane_run(const ldo_params* p, ldo_lane* o, lane_noise* n, double* frames, uint64_t frames_cap,
                     ldo_capture* cap, int lane_in_cap)
 {
@@ -1032,7 +1079,9 @@ static int lane_run(const ldo_params* p, ldo_lane* o, lane_noise* n, double* fra
     L.v = p->randomize_v ? c->vsigma * noise_init_gauss(n, 0, o) : p->v0;
     o->v0_instance = L.v;
     lane_energy_force(&L, c, model);
-    stats_init(p, o, &L, NULL, NULL, 0);
+    lane_touch_ctx tctx = {st, n, c, p};
+    touch_fn touch = (model == 0) ? lane_touch : NULL;
+    stats_init(p, o, &L, touch, &tctx, 0);
     if (p->overdamped) L.R = c->var * noise_init_gauss(n, 1, o);
     uint64_t nf = 0;
     lane_record(p, c, &L, frames, frames_cap, nf++, 1);
@@ -1084,7 +1133,7 @@ static int lane_run(const ldo_params* p, ldo_lane* o, lane_noise* n, double* fra
         lane_param(&L, c, model);
         lane_energy_force(&L, c, model);
         if (!p->overdamped) L.v = L.v + (c->minv * L.F) * c->halfdt;
-        stats_update(p, o, &L, iStep, NULL, NULL, 0, cap ? &cr : NULL);
+        stats_update(p, o, &L, iStep, touch, &tctx, 0, cap ? &cr : NULL);
         if (iStep % F == 0) lane_record(p, c, &L, frames, frames_cap, nf++, 1);
         /* double barrier: crossings of the intermediate-well bottom xc (a local
          * barrier change that is not a periodic-cell change) split the

@@ -76,6 +76,10 @@ typedef struct ldo_lane {
     /* noise consumption (lane mode) */
     uint64_t n_gauss, n_flip, n_catch;
     int error;
+    /* FPT1 barrier log (single barrier): states and ids of the two barriers
+     * around the well the last FPT1 started from */
+    int fpt_init_index[2];
+    double fpt_init_state[2];
 } ldo_lane;
 
 /* Optional per-lane capture of the first `cap` values of every event list. */
@@ -83,6 +87,7 @@ typedef struct ldo_capture {
     uint64_t cap;
     double *fpt, *fpt2, *tpt, *tpt2, *ticks;  /* [lane][cap] */
     double *cg;                               /* [cg][lane][cap] */
+    double *inst, *init;                      /* FPT1 barrier logs, [lane][cap] */
 } ldo_capture;
 
 /* Bank refill callback: write n doubles of stream `which` (0 = SIMD

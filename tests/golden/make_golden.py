@@ -74,6 +74,12 @@ def main():
                 "cg_n": [len(c["list"]) for c in L["cg"]],
                 "cg_first": [c["list"][:10].tolist() for c in L["cg"]],
                 "cg_sizes": L["cg_sizes"].tolist(),
+                # FPT1 barrier logs: states of the barrier crossed / under the particle
+                "inst_first": L["instant_barrier"][:20].tolist(), "init_first": L["initial_barrier"][:20].tolist(),
+                "inst_n": len(L["instant_barrier"]), "init_n": len(L["initial_barrier"]),
+                "inst_sum": float(sum(L["instant_barrier"].tolist())),
+                "init_sum": float(sum(L["initial_barrier"].tolist())),
+                "fpt_init_state": list(L["fpt_init_state"]), "fpt_init_index": list(L["fpt_init_index"]),
                 # per-lane means exactly as the runner computes them (STATISTICS::AVERAGE)
                 "mean_fpt": float(sum(L["fpt"].tolist()) / len(L["fpt"])) if len(L["fpt"]) else 0.0,
             })

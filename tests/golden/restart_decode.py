@@ -144,9 +144,16 @@ def decode(path, overdamped=True, tpt=True, fptcg=True):
     if overdamped:
         tail = r.vec(W)
     assert r.i32() == 9990
+    trailer = None
+    if r.pos < len(r.buf):
+        # files of the MI355X build carry their own checkpoint image after the end marker
+        assert r.u64() == 0x324B43504350444C
+        n = r.u64()
+        trailer = r.buf[r.pos:r.pos + n]
+        r.pos += n
     assert r.pos == len(r.buf), (r.pos, len(r.buf))
     return {"header": h, "lanes": lanes, "barrier_state": states, "barrier_step": steps,
-            "barrier_min_id": min_id, "r_prev": tail}
+            "barrier_min_id": min_id, "r_prev": tail, "trailer": trailer}
 
 
 class _Writer:

@@ -60,6 +60,7 @@ class Lane(C.Structure):
         ("xc_first_step", C.c_uint64), ("xc_last_step", C.c_uint64), ("steps_done", C.c_uint64),
         ("n_gauss", C.c_uint64), ("n_flip", C.c_uint64), ("n_catch", C.c_uint64),
         ("error", C.c_int),
+        ("fpt_init_index", C.c_int * 2), ("fpt_init_state", C.c_double * 2),
     ]
 
 
@@ -67,7 +68,8 @@ class Capture(C.Structure):
     _fields_ = [("cap", C.c_uint64),
                 ("fpt", C.POINTER(C.c_double)), ("fpt2", C.POINTER(C.c_double)),
                 ("tpt", C.POINTER(C.c_double)), ("tpt2", C.POINTER(C.c_double)),
-                ("ticks", C.POINTER(C.c_double)), ("cg", C.POINTER(C.c_double))]
+                ("ticks", C.POINTER(C.c_double)), ("cg", C.POINTER(C.c_double)),
+                ("inst", C.POINTER(C.c_double)), ("init", C.POINTER(C.c_double))]
 
 
 class Record(C.Structure):

@@ -24,7 +24,7 @@ class NumpyBanks:
             arr[:] = self.rng[which].random(n)
 
 
-def run_rank_recorded(oracle, p, W, seed, bmin=-300, bn=600, cap=None, bank=4096, trials=1):
+def run_rank_recorded(oracle, p, W, seed, bmin=-300, bn=600, cap=None, bank=4096, trials=1, capn=0):
     """Run one rank of W lanes; return lanes, frames [frame][lane] and the
     role-tagged per-lane streams.  With trials > 1 the rank first runs
     trials - 1 unrecorded trials (its streams and its lanes' initial barrier
@@ -47,11 +47,20 @@ def run_rank_recorded(oracle, p, W, seed, bmin=-300, bn=600, cap=None, bank=4096
     binit = np.full((2, W, bn), -1, np.int8)
     rec = ol.Record(cap, ol.dptr(rec_g), ol.dptr(rec_f), ol.dptr(rec_c), bmin, bn,
                     binit.ctypes.data_as(C.POINTER(C.c_int8)))
-    oracle.ldo_rank_integrate(st, lanes, ol.dptr(frames), nfr, C.byref(nf), None, C.byref(rec))
+    capture, cbufs = None, None
+    if capn:
+        # the raw event lists of the recorded trial, first capn entries of each
+        cbufs = {k: np.zeros((W, capn)) for k in ("fpt", "fpt2", "tpt", "tpt2", "ticks", "inst", "init")}
+        cbufs["cg"] = np.zeros((max(p.n_cg, 1), W, capn))
+        capture = ol.Capture(capn, *[ol.dptr(cbufs[k]) for k in ("fpt", "fpt2", "tpt", "tpt2", "ticks", "cg", "inst",
+                                                                  "init")])
+    oracle.ldo_rank_integrate(st, lanes, ol.dptr(frames), nfr, C.byref(nf), C.byref(capture) if capn else None,
+                              C.byref(rec))
     oracle.ldo_rank_destroy(st)
     streams = {
         "gauss": rec_g, "flip": rec_f, "catch": rec_c, "binit": binit, "bmin": bmin,
         "n_gauss": [lanes[i].n_gauss for i in range(W)], "n_flip": [lanes[i].n_flip for i in range(W)],
         "n_catch": [lanes[i].n_catch for i in range(W)],
+        "capture": cbufs,
     }
     return lanes, frames, streams

@@ -15,6 +15,7 @@ HEADER = os.path.join(REPO, "include", "ldpcp.h")
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"static inline[^{]*\{.*?\n\}", "", text, flags=re.S)  # header-only helpers
     return sorted(set(re.findall(r"\b(ldpcp_[a-z0-9_]+)\s*\(", text)))
 
 

@@ -442,3 +442,77 @@ def test_executable_continues_a_reference_restart(tmp_path):
     total = sum(sum(int(x) for x in ln.split()[3:4]) for ln in
                 [ln for ln in (tmp_path / "out_stateCount.log").read_text().splitlines() if ln.startswith("# state")])
     assert total == sum(int(ln.split()[3]) for ln in st["outputs"]["stateCount.log"]) + 16 * 600000
+
+
+def test_executable_writes_the_reference_restart_layout(tmp_path):
+    """WRITERESTART of the single-barrier program writes the reference's own
+    layout (TrajectoryData::WriteRestartFile, langevinDynamicsSingleBarrier.h:385-576):
+    it decodes with the decoder pinned on the reference's recorded files, every
+    lane's event lists average to the .dat values the run wrote, its x frames
+    end at its position; this program's exact image follows the end marker.
+    Cut after the end marker, the file alone restarts the run: with
+    NUMBEROFSTEPS 0 the outputs are the writing run's."""
+    import sys
+    sys.path.insert(0, GOLD)
+    import restart_decode as R
+    base = SINGLE_INP.format(over="true")
+    for k, v in dict(WRITERESTART="true", OUTPUTRESTART="rs", WRITEX="false", NUMBEROFRUNS="2").items():
+        base = re.sub(rf"(?m)^{k} .*$", f"{k} {v}", base)
+    ranks, W, trials = 2, 4, 2
+    exe = os.path.join(BIN, "langevinDynamicsSingleBarrier")
+    (tmp_path / "run.inp").write_text(base)
+    print("[gpu exe reference layout] write", flush=True)
+    r = subprocess.run([exe, "-i", "run.inp", "--ranks", str(ranks), "--simd-width", str(W)], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "Error" not in r.stderr, r.stderr[-2000:]
+    dat = {k: np.fromfile(tmp_path / f"out_{k}.dat", "<f8") for k in
+           ("firstPassageTime", "firstPassageTime2", "transitionPathTime", "transitionPathTime2",
+            "firstPassageTimeCG2_0000_0002")}
+
+    def mean(v):
+        s = 0.0
+        for x in v:
+            s += x
+        return s / len(v) if len(v) else 0.0
+
+    for rk in range(ranks):
+        for tr in range(trials):
+            path = tmp_path / f"rs_{rk:05d}_{tr:05d}.restart"
+            d = R.decode(str(path))
+            h = d["header"]
+            assert (h["signature"], h["simd_width"], h["rank"], h["nsteps_total"]) == (2345, W, rk, 200000)
+            assert d["trailer"] is not None
+            for ch, L in enumerate(d["lanes"]):
+                g = rk * trials * W + tr * W + ch      # rank-major, then trial, then lane
+                assert mean(L["fpt"]) == dat["firstPassageTime"][g]
+                assert mean(L["fpt2"]) == dat["firstPassageTime2"][g]
+                assert mean(L["tpt"]) == dat["transitionPathTime"][g]
+                assert mean(L["tpt2"]) == dat["transitionPathTime2"][g]
+                assert mean(L["cg"][0]["list"]) == dat["firstPassageTimeCG2_0000_0002"][g]
+                assert len(L["instant_barrier"]) == len(L["initial_barrier"]) == len(L["fpt"])
+                assert set(np.unique(L["instant_barrier"])) <= {0.0, 1.0}
+                assert len(L["traj_x"]) == 200000 // 100 + 1 and L["traj_x"][-1] == L["x"]
+                assert L["state_count0"] + L["state_count1"] == 200001
+            assert d["barrier_min_id"] <= -100 and d["barrier_state"].shape[0] >= 201
+            assert set(np.unique(d["barrier_state"])) <= {0, 1}
+    # the reference layout alone: cut the file after the end marker
+    for f in os.listdir(tmp_path):
+        if f.endswith(".restart"):
+            d = R.decode(str(tmp_path / f))
+            data = (tmp_path / f).read_bytes()
+            (tmp_path / f).write_bytes(data[:len(data) - len(d["trailer"]) - 16])
+            assert R.decode(str(tmp_path / f))["trailer"] is None
+    before = {k: v.copy() for k, v in dat.items()}
+    msd_before = np.loadtxt(tmp_path / "out_msd.log", comments="#")
+    text = base
+    for k, v in dict(WRITERESTART="false", READRESTART="true", INPUTRESTART="rs", NUMBEROFSTEPS="0",
+                     NUMBEROFPREVIOUSSTEPS="200000").items():
+        text = re.sub(rf"(?m)^{k} .*$", f"{k} {v}", text)
+    (tmp_path / "run.inp").write_text(text)
+    print("[gpu exe reference layout] read", flush=True)
+    r = subprocess.run([exe, "-i", "run.inp", "--ranks", str(ranks), "--simd-width", str(W)], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "Error" not in r.stderr, r.stderr[-2000:]
+    for k, v in before.items():
+        np.testing.assert_array_equal(np.fromfile(tmp_path / f"out_{k}.dat", "<f8"), v, err_msg=k)
+    np.testing.assert_array_equal(np.loadtxt(tmp_path / "out_msd.log", comments="#"), msd_before)

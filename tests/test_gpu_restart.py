@@ -22,6 +22,7 @@ import pytest
 
 from langevindynamicsforpcp_amd import LdpcpError
 from langevindynamicsforpcp_amd import _native as N
+from langevindynamicsforpcp_amd.integrator import read_checkpoint
 from streams import run_rank_recorded
 from test_gpu_run import COUNTS, EXACT, to_engine
 from test_oracle_modes import make
@@ -42,6 +43,7 @@ def lane_dict(lane):
 def engine(name, nsteps, n, seed, spl):
     le = to_engine(make(name, nsteps=nsteps, save_freq=100, save_all=1), n, seed=seed)
     le.tick_capacity = 4096
+    le.event_capacity = 4096
     le.steps_per_launch = spl
     return le
 
@@ -71,6 +73,8 @@ def test_gpu_resume_continues_bit_exact(name):
         np.testing.assert_array_equal(x, y)
     lags = np.array([0, 1, 2, 5, 10, 50, 200, 600], np.uint64)
     np.testing.assert_array_equal(second.msd(lags), full.msd(lags))
+    # the whole device state, event lists included, byte for byte
+    assert second.Checkpoint() == full.Checkpoint()
     for le in (full, first, second):
         le.close()
 
@@ -160,4 +164,37 @@ def test_gpu_same_initial_barrier_second_trial_matches_oracle(oracle):
     for i in range(W):
         for f in EXACT + COUNTS:
             assert getattr(got[i], f) == getattr(lanes[i], f), (i, f)
+    le.close()
+
+
+@pytest.mark.parametrize("name", ["single_over", "single_under", "single_eq_false"])
+def test_gpu_event_lists_match_oracle(oracle, name):
+    """The raw event lists a restart file in the reference's layout needs
+    (FPT1, FPT2, TPT1, TPT2, FPTCG and the FPT1 barrier logs of the crossed and
+    the left barrier, TrajectoryData, langevinDynamicsSingleBarrier.h:160-235),
+    kept on the device, equal the oracle's lists entry for entry (replayed noise)."""
+    W, capn = 8, 8192
+    p = make(name, nsteps=60000, save_freq=100)
+    print(f"[gpu event lists] {name}", flush=True)
+    lanes, frames, s = run_rank_recorded(oracle, p, W, seed=19, capn=capn)
+    ng, nf = min(s["n_gauss"]), min(s["n_flip"])
+    le = to_engine(p, W)
+    le.barrier_window = 1024
+    le.event_capacity = capn
+    le.SetReplay(s["gauss"][:, :ng].T, s["flip"][:, :nf].T, s["catch"], s["binit"], s["bmin"])
+    le.Integrate(0)
+    img = read_checkpoint(le.Checkpoint())
+    assert img["header"].events_complete == 1 and img["header"].n_lists == 6 + p.n_cg
+    cap = s["capture"]
+    names = ["fpt", "fpt2", "tpt", "tpt2", "inst", "init"]
+    for i in range(W):
+        got = img["events"][i]
+        assert list(img["lanes"][i].fpt_init_index) == list(lanes[i].fpt_init_index)
+        assert list(img["lanes"][i].fpt_init_state) == list(lanes[i].fpt_init_state)
+        for k, key in enumerate(names):
+            assert len(got[k]) <= capn
+            np.testing.assert_array_equal(got[k], cap[key][i, :len(got[k])], err_msg=f"{key} lane {i}")
+        for j in range(p.n_cg):
+            np.testing.assert_array_equal(got[6 + j], cap["cg"][j, i, :len(got[6 + j])], err_msg=f"cg{j}")
+    assert sum(len(img["events"][i][4]) for i in range(W)) > 0, "no FPT1 event: the barrier logs are not exercised"
     le.close()

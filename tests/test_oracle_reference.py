@@ -94,9 +94,10 @@ def _run_rank(oracle, rank, seed, nsteps, capn=0):
     cap = None
     bufs = None
     if capn:
-        bufs = {k: np.zeros((4, capn)) for k in ("fpt", "fpt2", "tpt", "tpt2", "ticks")}
+        bufs = {k: np.zeros((4, capn)) for k in ("fpt", "fpt2", "tpt", "tpt2", "ticks", "inst", "init")}
         bufs["cg"] = np.zeros((7, 4, capn))
-        cap = ol.Capture(capn, *[ol.dptr(bufs[k]) for k in ("fpt", "fpt2", "tpt", "tpt2", "ticks", "cg")])
+        cap = ol.Capture(capn, *[ol.dptr(bufs[k]) for k in ("fpt", "fpt2", "tpt", "tpt2", "ticks", "cg", "inst",
+                                                             "init")])
     oracle.ldo_rank_integrate(st, lanes, ol.dptr(frames), nfr, C.byref(nf), C.byref(cap) if cap else None, None)
     renew = (C.c_int64 * 3)()
     off = (C.c_uint64 * 3)()
@@ -120,8 +121,10 @@ def test_rank_mode_reproduces_recorded_frames(oracle, golden, rank):
         assert lanes[i].state0_instance == ref["state0_instance"]
         np.testing.assert_array_equal(frames[:, i], frames_ref[rank * 4 + i, :21])
         # the first recorded events of every list, bit for bit
-        for key, n_attr in (("fpt", "fpt_n"), ("fpt2", "fpt2_n"), ("tpt", "tpt_n"), ("tpt2", "tpt2_n")):
+        for key, n_attr in (("fpt", "fpt_n"), ("fpt2", "fpt2_n"), ("tpt", "tpt_n"), ("tpt2", "tpt2_n"),
+                            ("inst", "fpt_n"), ("init", "fpt_n")):
             k = min(int(getattr(lanes[i], n_attr)), len(ref[key + "_first"]), 64)
+            assert k > 0, key
             np.testing.assert_array_equal(bufs[key][i, :k], ref[key + "_first"][:k])
 
 
@@ -144,3 +147,5 @@ def test_rank_mode_full_run_matches_restart(oracle, golden, rank):
             ref["fpt_n"], ref["fpt2_n"], ref["tpt_n"], ref["tpt2_n"], ref["ticks_n"])
         assert [lanes[i].cg_n[j] for j in range(7)] == ref["cg_n"]
         assert lanes[i].fpt_sum / lanes[i].fpt_n == ref["mean_fpt"]
+        assert list(lanes[i].fpt_init_state) == ref["fpt_init_state"]
+        assert list(lanes[i].fpt_init_index) == ref["fpt_init_index"]
