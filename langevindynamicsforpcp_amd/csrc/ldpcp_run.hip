@@ -9,6 +9,7 @@
 // lives in VGPRs for a whole launch; statistics and the per-trajectory barrier
 // table live in HBM and are touched only on the rare crossing events.
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -178,25 +179,67 @@ struct Lane {
     double dh[2], ds[2][2], di[2][2];               // double barrier
     double well, periodic, well_prev, periodic_prev, xp, local, local_prev;
     bool ent_well, ent_period, ent_local;
+    int wbase;  // barrier id of the window's first entry
     uint64_t cnt0, cnt1;
 };
 
+// Heights, slopes and intercepts of both barrier states, which every step
+// selects from by the current state.  A selection between two kernel-argument
+// fields is compiled as one load from the selected address, a memory round trip
+// on the critical path of every step; each value here passes through an empty
+// asm statement, so it stays an opaque register value loaded once per launch.
+struct StateParams {
+    double H[2], S[2];                          // single barrier [state]
+    double dh[2][2], ds[2][2][2], di[2][2][2];  // double barrier [state][barrier][side]
+    double p_flip[2];                           // per-step flip probability [state] (EQUALPROB false)
+};
+
+__device__ __forceinline__ double in_register(double v)
+{
+    asm("" : "+v"(v));
+    return v;
+}
+
+__device__ __forceinline__ StateParams state_params(const RunConst& c, const Flags& f)
+{
+    StateParams sp;
+    sp.p_flip[0] = in_register(c.p_asym0);
+    sp.p_flip[1] = in_register(c.p_asym1);
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+        if (f.model == 0) {
+            sp.H[st] = in_register(c.H[st]);
+            sp.S[st] = in_register(c.S[st]);
+            continue;
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            sp.dh[st][b] = in_register(c.Hd[b][st]);
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                sp.ds[st][b][side] = in_register(c.slope[b][st][side]);
+                sp.di[st][b][side] = in_register(c.icept[b][st][side]);
+            }
+        }
+    }
+    return sp;
+}
+
 // UpdateBarrierStateEtaParamter (Single.h:2154-2165; Double.h:2737-2757)
-__device__ __forceinline__ void lane_param(Lane& s, const RunConst& c, const Flags& f, bool count = true)
+__device__ __forceinline__ void lane_param(Lane& s, const StateParams& sp, const Flags& f, bool count = true)
 {
     const bool one = (s.state == 1.0);
     if (f.model == 0) {
-        s.height = one ? c.H[1] : c.H[0];
-        s.slope_pos = one ? c.S[1] : c.S[0];
+        s.height = one ? sp.H[1] : sp.H[0];
+        s.slope_pos = one ? sp.S[1] : sp.S[0];
     } else {
-        const int st = one ? 1 : 0;
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            s.dh[b] = c.Hd[b][st];
+            s.dh[b] = one ? sp.dh[1][b] : sp.dh[0][b];
 #pragma unroll
             for (int side = 0; side < 2; ++side) {
-                s.ds[b][side] = c.slope[b][st][side];
-                s.di[b][side] = c.icept[b][st][side];
+                s.ds[b][side] = one ? sp.ds[1][b][side] : sp.ds[0][b][side];
+                s.di[b][side] = one ? sp.di[1][b][side] : sp.di[0][b][side];
             }
         }
     }
@@ -354,25 +397,104 @@ __device__ __forceinline__ int barrier_init_state(const RunConst& c, const Flags
     return unit_f64(w.w[0]) < c.p_create ? 0 : 1;
 }
 
-// Barrier table entry for (array, id): valid bit, state bit, 62-bit time step.
-__device__ __forceinline__ uint64_t* win_entry(const RunConst& c, const RunPtrs& P, uint32_t t, int array, int id,
-                                               int& err)
+// A trajectory's barrier table: the window of wb entries per barrier array
+// (valid bit, state bit, 62-bit time step) from barrier id wbase on, optionally
+// held in a per-trajectory LDS cache, direct-mapped with kWinSlots entries per
+// array, that writes an entry back to the window when it is evicted and at the
+// end of the launch (win_flush).  With the cache a hand-over between nearby
+// barriers touches LDS only and never waits for global memory; every access of
+// a launch goes through the same view, so the cache is the table's current
+// state until it is flushed.
+constexpr int kWinSlots = 16;
+constexpr int kWinEmpty = INT_MIN;  // cache slot holds no barrier
+
+struct Window {
+    uint64_t* win;   // this trajectory's arrays: win[array * wb + k]
+    int wb, wbase;
+    bool cached;     // a constant of the kernel, so that the cache's code folds away when unused
+    uint64_t* ce;    // LDS cache entries [array][slot][64 lanes]
+    int* cid;        // their barrier ids (kWinEmpty: none), same layout
+    uint8_t* dirty;  // 1: entry changed since it was read from the window
+    uint32_t lane;
+};
+
+__device__ __forceinline__ Window make_window(const RunConst& c, const RunPtrs& P, uint32_t t, int wbase,
+                                              uint64_t* ce = nullptr, int* cid = nullptr, uint8_t* dirty = nullptr)
 {
-    const int k = id - P.wbase[t];
-    if (k < 0 || k >= c.wb) {
-        err = 3;
-        return nullptr;
-    }
-    return P.win + ((uint64_t)t * 2 + array) * c.wb + k;
+    return {P.win + (uint64_t)t * 2 * c.wb, c.wb, wbase, ce != nullptr, ce, cid, dirty, threadIdx.x & 63u};
 }
 
-__device__ __forceinline__ uint64_t win_load(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
-                                             int array, int id, uint64_t* e, int& err)
+__device__ __forceinline__ int win_cache_slot(const Window& w, int array, int id)
 {
-    uint64_t v = *e;
+    return (array * kWinSlots + (id & (kWinSlots - 1))) * 64 + (int)w.lane;
+}
+
+// writes the entry in cache slot sl (barrier `id` of `array`) back if it changed
+__device__ __forceinline__ void win_evict(const Window& w, int array, int sl, int id)
+{
+    if (id != kWinEmpty && w.dirty[sl]) w.win[array * w.wb + (id - w.wbase)] = w.ce[sl];
+}
+
+__device__ __forceinline__ void win_flush(const Window& w)
+{
+    if (!w.cached) return;
+    for (int a = 0; a < 2; ++a)
+        for (int j = 0; j < kWinSlots; ++j) {
+            const int sl = (a * kWinSlots + j) * 64 + (int)w.lane;
+            win_evict(w, a, sl, w.cid[sl]);
+        }
+}
+
+// window index of barrier `id`, or -1 with err = 3 when the trajectory has left the window
+__device__ __forceinline__ int win_slot(const Window& w, int id, int& err)
+{
+    const int k = id - w.wbase;
+    if (k < 0 || k >= w.wb) {
+        err = 3;
+        return -1;
+    }
+    return k;
+}
+
+__device__ __forceinline__ uint64_t win_get(const Window& w, int array, int id, int k)
+{
+    if (w.cached) {
+        const int sl = win_cache_slot(w, array, id);
+        const int held = w.cid[sl];
+        if (held == id) return w.ce[sl];
+        win_evict(w, array, sl, held);
+        const uint64_t v = w.win[array * w.wb + k];
+        w.cid[sl] = id;
+        w.ce[sl] = v;
+        w.dirty[sl] = 0;
+        return v;
+    }
+    return w.win[array * w.wb + k];
+}
+
+__device__ __forceinline__ void win_put(const Window& w, int array, int id, int k, uint64_t v)
+{
+    if (w.cached) {
+        const int sl = win_cache_slot(w, array, id);
+        const int held = w.cid[sl];
+        if (held != id) win_evict(w, array, sl, held);
+        w.cid[sl] = id;
+        w.ce[sl] = v;
+        w.dirty[sl] = 1;
+        return;
+    }
+    w.win[array * w.wb + k] = v;
+}
+
+// entry of barrier (array, id) at window index k, created (its state at step 0)
+// if this trajectory has not touched it yet
+__device__ __forceinline__ uint64_t win_load(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
+                                             const Window& w, int array, int id, int k, int& err)
+{
+    uint64_t v = win_get(w, array, id, k);
     if (!(v & kValid)) {
-        v = kValid | (barrier_init_state(c, f, P, t, array, id, err) ? kState : 0ull);  // created at step 0
-        *e = v;
+        v = kValid | (barrier_init_state(c, f, P, t, array, id, err) ? kState : 0ull);
+        win_put(w, array, id, k, v);
     }
     return v;
 }
@@ -394,28 +516,28 @@ __device__ __forceinline__ void arm_indices(int idx[2], const Lane& s)
 
 // Stored state of barrier `id` of the single barrier's array, created (its
 // state at step 0) if this trajectory has not touched it yet.
-__device__ __forceinline__ int stored_state(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t, int id,
-                                            int& err)
+__device__ __forceinline__ int stored_state(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
+                                            const Window& w, int id, int& err)
 {
-    uint64_t* e = win_entry(c, P, t, 0, id, err);
-    if (!e) return 0;
-    return (win_load(c, f, P, t, 0, id, e, err) & kState) ? 1 : 0;
+    const int k = win_slot(w, id, err);
+    if (k < 0) return 0;
+    return (win_load(c, f, P, t, w, 0, id, k, err) & kState) ? 1 : 0;
 }
 
 // FPT1 arming (Single.h:2364-2387; at step 0 :2184-2212): the barriers either
 // side of the well the particle sits in, the one under the particle with its
 // current state, the other with its stored state.
 __device__ __forceinline__ void fpt1_arm(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
-                                         ldpcp_lane* o, const Lane& s, int& err)
+                                         const Window& w, ldpcp_lane* o, const Lane& s, int& err)
 {
     const int pidx = (int)s.periodic;
     if (s.well == s.periodic) {
         o->fpt_init_state[0] = s.state;
         o->fpt_init_index[0] = pidx;
-        o->fpt_init_state[1] = (double)stored_state(c, f, P, t, pidx + 1, err);
+        o->fpt_init_state[1] = (double)stored_state(c, f, P, t, w, pidx + 1, err);
         o->fpt_init_index[1] = pidx + 1;
     } else {
-        o->fpt_init_state[0] = (double)stored_state(c, f, P, t, pidx - 1, err);
+        o->fpt_init_state[0] = (double)stored_state(c, f, P, t, w, pidx - 1, err);
         o->fpt_init_index[0] = pidx - 1;
         o->fpt_init_state[1] = s.state;
         o->fpt_init_index[1] = pidx;
@@ -429,7 +551,7 @@ __device__ __forceinline__ void push_event(const RunConst& c, double* ev, int li
 }
 
 __device__ __forceinline__ void stats_init(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
-                                           ldpcp_lane* o, const Lane& s, int& err)
+                                           const Window& w, ldpcp_lane* o, const Lane& s, int& err)
 {
     if (c.b_wait) {
         o->wt_tick_prev = 0.0;
@@ -438,7 +560,7 @@ __device__ __forceinline__ void stats_init(const RunConst& c, const Flags& f, co
     if (c.b_fpt) {
         o->fpt_last_tick = 0.0;
         o->fpt_to_update = c.centre ? 0 : 1;
-        if (f.model == 0 && c.centre) fpt1_arm(c, f, P, t, o, s, err);
+        if (f.model == 0 && c.centre) fpt1_arm(c, f, P, t, w, o, s, err);
         o->fpt2_last_tick = 0.0;
         if (c.centre) {
             o->fpt2_to_update = 0;
@@ -476,7 +598,8 @@ __device__ __forceinline__ double duration(const RunConst& c, uint64_t iStep, do
 }
 
 __device__ __forceinline__ void stats_event(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t traj,
-                                            ldpcp_lane* o, const Lane& s, uint64_t iStep, double* ticks, int& err)
+                                            const Window& w, ldpcp_lane* o, const Lane& s, uint64_t iStep,
+                                            double* ticks, int& err)
 {
     const double t = c.dt * (double)iStep;
     double* const ev = c.ev_cap ? P.events + (uint64_t)traj * c.n_lists * c.ev_cap : nullptr;
@@ -505,7 +628,7 @@ __device__ __forceinline__ void stats_event(const RunConst& c, const Flags& f, c
             if (o->fpt_to_update && s.ent_period) {
                 o->fpt_to_update = 0;
                 o->fpt_last_tick = t;
-                fpt1_arm(c, f, P, traj, o, s, err);
+                fpt1_arm(c, f, P, traj, w, o, s, err);
             }
             if (!o->fpt_to_update && s.ent_well) {
                 o->fpt_to_update = 1;
@@ -605,7 +728,8 @@ __device__ __forceinline__ void record(const RunConst& c, const Flags& f, const 
     }
 }
 
-__device__ __forceinline__ void load_lane(Lane& s, const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t)
+__device__ __forceinline__ void load_lane(Lane& s, const StateParams& sp, const Flags& f, const RunPtrs& P,
+                                          uint32_t t)
 {
     s.x = P.x[t];
     s.v = P.v[t];
@@ -620,8 +744,9 @@ __device__ __forceinline__ void load_lane(Lane& s, const RunConst& c, const Flag
     s.local = P.local[t];
     s.cnt0 = P.cnt0[t];
     s.cnt1 = P.cnt1[t];
+    s.wbase = P.wbase[t];
     s.ent_well = s.ent_period = s.ent_local = false;
-    lane_param(s, c, f, false);  // rebuild heights and slopes from the stored state
+    lane_param(s, sp, f, false);  // rebuild heights and slopes from the stored state
 }
 
 __device__ __forceinline__ void store_lane(const Lane& s, const RunPtrs& P, uint32_t t)
@@ -665,14 +790,16 @@ __global__ void __launch_bounds__(256) run_init_kernel(RunConst c, RunPtrs P)
         s.state = c.initial_state;
     }
     o->state0_instance = s.state;
-    P.wbase[t] = (int)s.periodic - c.wb / 2;
+    s.wbase = (int)s.periodic - c.wb / 2;
+    P.wbase[t] = s.wbase;
+    const Window w = make_window(c, P, t, s.wbase);
     {
         const int a = f.model == 0 ? 0 : (int)s.local;
-        uint64_t* e = win_entry(c, P, t, a, (int)s.periodic, err);
-        if (e) *e = kValid | ((int)s.state ? kState : 0ull);
+        const int k = win_slot(w, (int)s.periodic, err);
+        if (k >= 0) win_put(w, a, (int)s.periodic, k, kValid | ((int)s.state ? kState : 0ull));
     }
     s.cnt0 = s.cnt1 = 0;
-    lane_param(s, c, f);
+    lane_param(s, state_params(c, f), f);
     double gv = 0.0, gr = 0.0;
     if (!f.replay) box_muller_f64(w0.w[0], w0.w[1], gv, gr);
     if (c.randomize_v) {
@@ -684,7 +811,7 @@ __global__ void __launch_bounds__(256) run_init_kernel(RunConst c, RunPtrs P)
     }
     o->v0_instance = s.v;
     lane_energy_force(s, c, f);
-    stats_init(c, f, P, t, o, s, err);
+    stats_init(c, f, P, t, w, o, s, err);
     if (f.over) {
         const double g = f.replay ? P.g[ng * c.n_traj + t] : gr;
         ng++;
@@ -713,8 +840,8 @@ __global__ void __launch_bounds__(256) run_init_kernel(RunConst c, RunPtrs P)
 constexpr int kNoisePhilox = 0, kNoiseReplay = 1, kNoiseBuffered = 2;
 constexpr int kStage = 32;  // steps per LDS stage of the buffered noise
 // s_waitcnt immediate (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4,
-// lgkmcnt 11:8) that waits for vmcnt = 0 and leaves the other counters free
-constexpr int kWaitVmcnt0 = 0x0F70;
+// lgkmcnt 11:8) that waits for vmcnt = 0 and lgkmcnt = 0 and leaves expcnt free
+constexpr int kWaitVmLgkm0 = 0x0070;
 
 // One buffered noise record per trajectory-step: Gaussian (lo, hi words) and
 // the 32-bit flip word.
@@ -732,74 +859,44 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     constexpr Flags f{MODEL, OVER, TAU0, REPLAY};
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= c.n_traj) return;
-    ldpcp_lane* o = P.lanes + t;
-    if (o->error) return;
+    ldpcp_lane* const og = P.lanes + t;
+    if (og->error) return;
+    // the statistics record the events update: in LDS for the buffered launch (one
+    // wavefront per workgroup, few trajectories, where each event's dependent
+    // reads are on the step's critical path), else in place
+    // the buffered launch (one wavefront per workgroup, few trajectories) keeps the
+    // statistics record and the barrier table in LDS: an event's reads and a
+    // hand-over's table lookups are on the step's critical path
+    constexpr bool kLocal = (SRC == kNoiseBuffered);
+    __shared__ ldpcp_lane lane_stats[64];
+    __shared__ uint64_t win_cache[2 * kWinSlots * 64];
+    __shared__ int win_cache_id[2 * kWinSlots * 64];
+    __shared__ uint8_t win_cache_dirty[2 * kWinSlots * 64];
+    ldpcp_lane* o = og;
+    if constexpr (kLocal) {
+        lane_stats[threadIdx.x] = *og;
+        o = &lane_stats[threadIdx.x];
+        for (int k = 0; k < 2 * kWinSlots; ++k) win_cache_id[k * 64 + threadIdx.x] = kWinEmpty;
+    }
     int err = 0;
+    const StateParams sp = state_params(c, f);
     Lane s;
-    load_lane(s, c, f, P, t);
-    StepNoise sn;
-    sn.q = 0;
-    sn.g1 = 0.0;
-    sn.w3 = 0;
+    load_lane(s, sp, f, P, t);
+    Window w = make_window(c, P, t, s.wbase);
+    if constexpr (kLocal) {
+        w.cached = true;
+        w.ce = win_cache;
+        w.cid = win_cache_id;
+        w.dirty = win_cache_dirty;
+    }
     uint64_t n_catch = o->n_catch;
     // frames every save_freq steps: a countdown instead of a 64-bit modulo per step
     uint64_t to_save = c.save_freq - s0 % c.save_freq;
     uint64_t frame = s0 / c.save_freq + 1;
-    // replay: the noise of the next kAhead steps is in flight while a step runs
-    // (indices clamp to s1, which the launch keeps above s0)
-    constexpr int kAhead = 4;
-    double gq[kAhead], uq[kAhead];
-    if constexpr (REPLAY) {
-#pragma unroll
-        for (int d = 0; d < kAhead; ++d)
-            replay_noise(c, f, P, t, (s0 + 1 + d < s1) ? s0 + 1 + d : s1, gq[d], uq[d]);
-    }
-    // buffered noise: stage j of the launch holds steps s0 + 1 + 32 j ...; the
-    // launch is one wavefront per workgroup (64 threads), so a stage's
-    // __syncthreads only waits for this wavefront's own loads
-    [[maybe_unused]] const uint32_t lane = threadIdx.x;
-    typedef uint4 StageBuf[2][kStage][64];
-    [[maybe_unused]] StageBuf* stage = nullptr;
-    if constexpr (SRC == kNoiseBuffered) {
-        __shared__ uint4 stage_mem[2][kStage][64];
-        stage = &stage_mem;
-        const uint64_t len = s1 - s0;
-        const uint64_t m = len < kStage ? len : kStage;
-        for (uint64_t j = 0; j < m; ++j)
-            __builtin_amdgcn_global_load_lds((const void*)(noise + j * c.n_traj + t),
-                                             (__attribute__((address_space(3))) void*)&stage_mem[0][j][0], 16, 0, 0);
-    }
-    uint64_t iStep;
-    for (iStep = s0 + 1; iStep <= s1; ++iStep) {
-        double g, u;
-        if constexpr (SRC == kNoiseBuffered) {
-            const uint64_t k = iStep - s0 - 1;  // step within the launch
-            const uint32_t j = (uint32_t)(k % kStage), b = (uint32_t)((k / kStage) & 1);
-            if (j == 0) {
-                // this stage's direct-to-LDS loads have landed: they count in vmcnt
-                // only, which a workgroup barrier does not wait for
-                __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-                __syncthreads();
-                const uint64_t next = k + kStage, len = s1 - s0;
-                const uint64_t m = next >= len ? 0 : (len - next < kStage ? len - next : kStage);
-                for (uint64_t jj = 0; jj < m; ++jj)
-                    __builtin_amdgcn_global_load_lds((const void*)(noise + (next + jj) * c.n_traj + t),
-                                                     (__attribute__((address_space(3))) void*)&(*stage)[b ^ 1][jj][0],
-                                                     16, 0, 0);
-            }
-            buffered_noise((*stage)[b][j][lane], g, u);
-        } else if constexpr (REPLAY) {
-            g = gq[0];
-            u = uq[0];
-#pragma unroll
-            for (int d = 0; d + 1 < kAhead; ++d) {
-                gq[d] = gq[d + 1];
-                uq[d] = uq[d + 1];
-            }
-            replay_noise(c, f, P, t, (iStep + kAhead < s1) ? iStep + kAhead : s1, gq[kAhead - 1], uq[kAhead - 1]);
-        } else {
-            step_noise(c, f, P, t, iStep, sn, g, u);
-        }
+    uint64_t done = s0;  // last step completed
+
+    // One step with its noise (g, u); false ends the trajectory's launch (err set).
+    auto step = [&](uint64_t iStep, double g, double u) -> bool {
         if (f.over) {
             const double rf = c.var * g;
             s.x = fma(s.F, c.dtgm, s.x);
@@ -819,7 +916,7 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
                 if (u < c.p_flip) s.state = (s.state == 0.0) ? 1.0 : 0.0;
             } else {
                 const bool one = (s.state == 1.0);
-                if (u < (one ? c.p_asym1 : c.p_asym0)) s.state = one ? 0.0 : 1.0;
+                if (u < (one ? sp.p_flip[1] : sp.p_flip[0])) s.state = one ? 0.0 : 1.0;
             }
         }
         // hand-over to the barrier just entered (Single.h:2025-2068; Double.h:2466-2510)
@@ -827,11 +924,11 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         if (entering) {
             const int ap = f.model == 0 ? 0 : (int)s.local_prev;
             const int ac = f.model == 0 ? 0 : (int)s.local;
-            uint64_t* ep = win_entry(c, P, t, ap, (int)s.periodic_prev, err);
-            uint64_t* ec = win_entry(c, P, t, ac, (int)s.periodic, err);
-            if (err) break;
-            *ep = kValid | ((int)s.state ? kState : 0ull) | (iStep & kStepMask);
-            const uint64_t ev = win_load(c, f, P, t, ac, (int)s.periodic, ec, err);
+            const int kp = win_slot(w, (int)s.periodic_prev, err);
+            const int kc = win_slot(w, (int)s.periodic, err);
+            if (err) return false;
+            win_put(w, ap, (int)s.periodic_prev, kp, kValid | ((int)s.state ? kState : 0ull) | (iStep & kStepMask));
+            const uint64_t ev = win_load(c, f, P, t, w, ac, (int)s.periodic, kc, err);
             const double prev_state = (ev & kState) ? 1.0 : 0.0;
             if (f.tau0) {
                 s.state = prev_state;
@@ -845,36 +942,122 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
                 if (f.replay) {
                     if (n_catch >= c.catch_cap) {
                         err = 4;
-                        break;
+                        return false;
                     }
                     uc = P.c[(uint64_t)t * c.catch_cap + n_catch];
                 } else {
-                    const Philox4 w = philox4x32_10((uint32_t)n_catch, (uint32_t)(n_catch >> 32), c.traj_offset + t, 2u,
-                                                    c.key0, c.key1);
+                    const Philox4 w = philox4x32_10((uint32_t)n_catch, (uint32_t)(n_catch >> 32), c.traj_offset + t,
+                                                    2u, c.key0, c.key1);
                     uc = unit_f64(w.w[0]);
                 }
                 n_catch++;
                 s.state = (uc < pc) ? ((prev_state == 1.0) ? 0.0 : 1.0) : prev_state;
             }
-            if (err) break;
+            if (err) return false;
         }
-        lane_param(s, c, f);
+        lane_param(s, sp, f);
         lane_energy_force(s, c, f);
         if (!f.over) s.v = s.v + (c.minv * s.F) * c.halfdt;
         if (s.ent_period || s.ent_well) {
-            stats_event(c, f, P, t, o, s, iStep, P.ticks + (uint64_t)t * c.tick_cap, err);
-            if (err) break;
+            stats_event(c, f, P, t, w, o, s, iStep, P.ticks + (uint64_t)t * c.tick_cap, err);
+            if (err) return false;
         }
         if (--to_save == 0) {
             record(c, f, P, t, s, frame++);
             to_save = c.save_freq;
         }
+        done = iStep;
+        return true;
+    };
+
+    const uint64_t len = s1 - s0;
+    if constexpr (SRC == kNoiseBuffered) {
+        // Stage q of the launch (steps s0 + 1 + 32 q ...) sits in stage_a (q even) or
+        // stage_b (q odd), loaded by direct global->LDS loads while the stage before
+        // it runs.  The even and odd stages are separate code over separate arrays,
+        // so the compiler sees that a stage's LDS reads never touch the array the
+        // next stage's loads are landing in and waits for those loads only at the
+        // stage boundary.  The launch is one wavefront per workgroup (64 threads),
+        // so a boundary's __syncthreads waits for this wavefront alone.
+        __shared__ uint4 stage_a[kStage][64], stage_b[kStage][64];
+        const uint32_t lane = threadIdx.x;
+        auto load_stage = [&](uint4 (&buf)[kStage][64], uint64_t q) {
+            const uint64_t first = q * kStage;
+            const uint64_t m = first >= len ? 0 : (len - first < kStage ? len - first : kStage);
+            for (uint64_t j = 0; j < m; ++j)
+                __builtin_amdgcn_global_load_lds((const void*)(noise + (first + j) * c.n_traj + t),
+                                                 (__attribute__((address_space(3))) void*)&buf[j][0], 16, 0, 0);
+        };
+        // steps of stage q from buf; each step reads the next step's record, so
+        // the LDS latency overlaps a step
+        auto run_stage = [&](uint4 (&buf)[kStage][64], uint64_t q) -> bool {
+            const uint64_t first = q * kStage;
+            const uint32_t m = (uint32_t)(len - first < kStage ? len - first : kStage);
+            uint4 rec = buf[0][lane];
+            for (uint32_t j = 0; j < m; ++j) {
+                uint4 next = rec;
+                if (j + 1 < m) next = buf[j + 1][lane];
+                double g, u;
+                buffered_noise(rec, g, u);
+                if (!step(s0 + first + j + 1, g, u)) return false;
+                rec = next;
+            }
+            return true;
+        };
+        // stage q has landed (and every earlier LDS read is done): load stage q + 1
+        // into the other array, whose stage q - 1 has been read
+        auto boundary = [&]() {
+            __builtin_amdgcn_s_waitcnt(kWaitVmLgkm0);
+            __syncthreads();
+        };
+        load_stage(stage_a, 0);
+        for (uint64_t q = 0; q * kStage < len; q += 2) {
+            boundary();
+            load_stage(stage_b, q + 1);
+            if (!run_stage(stage_a, q)) break;
+            if ((q + 1) * kStage >= len) break;
+            boundary();
+            load_stage(stage_a, q + 2);
+            if (!run_stage(stage_b, q + 1)) break;
+        }
+    } else if constexpr (REPLAY) {
+        // the noise of the next kAhead steps is in flight while a step runs
+        // (indices clamp to s1, which the launch keeps above s0)
+        constexpr int kAhead = 4;
+        double gq[kAhead], uq[kAhead];
+#pragma unroll
+        for (int d = 0; d < kAhead; ++d)
+            replay_noise(c, f, P, t, (s0 + 1 + d < s1) ? s0 + 1 + d : s1, gq[d], uq[d]);
+        for (uint64_t iStep = s0 + 1; iStep <= s1; ++iStep) {
+            const double g = gq[0], u = uq[0];
+#pragma unroll
+            for (int d = 0; d + 1 < kAhead; ++d) {
+                gq[d] = gq[d + 1];
+                uq[d] = uq[d + 1];
+            }
+            replay_noise(c, f, P, t, (iStep + kAhead < s1) ? iStep + kAhead : s1, gq[kAhead - 1], uq[kAhead - 1]);
+            if (!step(iStep, g, u)) break;
+        }
+    } else {
+        StepNoise sn;
+        sn.q = 0;
+        sn.g1 = 0.0;
+        sn.w3 = 0;
+        for (uint64_t iStep = s0 + 1; iStep <= s1; ++iStep) {
+            double g, u;
+            step_noise(c, f, P, t, iStep, sn, g, u);
+            if (!step(iStep, g, u)) break;
+        }
     }
     o->n_catch = n_catch;
-    o->n_gauss += (iStep - 1) - s0;
-    if (!f.tau0) o->n_flip += (iStep - 1) - s0;
-    o->steps_done = iStep - 1;
+    o->n_gauss += done - s0;
+    if (!f.tau0) o->n_flip += done - s0;
+    o->steps_done = done;
     o->error = err;
+    if constexpr (kLocal) {
+        *og = *o;
+        win_flush(w);
+    }
     store_lane(s, P, t);
 }
 
@@ -932,11 +1115,12 @@ __global__ void __launch_bounds__(256) run_table_kernel(RunConst c, RunPtrs P, i
     const Flags f = run_flags(c);
     const int id = lo + k;
     int err = 0;
-    uint64_t* e = win_entry(c, P, t, array, id, err);
+    const Window w = make_window(c, P, t, P.wbase[t]);
+    const int kw = win_slot(w, id, err);
     int st;
     uint64_t step = 0;
-    if (e) {
-        const uint64_t v = win_load(c, f, P, t, array, id, e, err);
+    if (kw >= 0) {
+        const uint64_t v = win_load(c, f, P, t, w, array, id, kw, err);
         st = (v & kState) ? 1 : 0;
         step = v & kStepMask;
     } else {
