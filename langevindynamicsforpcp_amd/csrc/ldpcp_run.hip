@@ -32,6 +32,7 @@ struct RunConst {
     int32_t model, overdamped, tau_zero, equal_prob, same_init, centre;
     int32_t b_wait, b_fpt, b_tpt, b_cg, n_cg, save_all, randomize_v, randomize_state, replay, wb;
     int32_t cg_size[LDPCP_MAX_CG];
+    uint64_t t_flip, t_asym0, t_asym1;  // flip_threshold of p_flip, p_asym0, p_asym1
     double dt, minv, mhalf, halfdt, expmgdt, var, dtgm, L, Linv, c_lap, p_flip, p_asym0, p_asym1, K, p0,
         p_create;
     double H[2], S[2], w;
@@ -110,6 +111,9 @@ void make_const(const ldpcp_run_params& p, RunConst& c)
         }
         c.p0 = 1.0 / (1.0 + p.eq_constant);
     }
+    c.t_flip = flip_threshold(c.p_flip);
+    c.t_asym0 = flip_threshold(c.p_asym0);
+    c.t_asym1 = flip_threshold(c.p_asym1);
     c.p_create = 1.0 / (1.0 + p.eq_constant);
     c.Linv = 1.0 / c.L;
     if (p.model == 0) {
@@ -192,9 +196,16 @@ struct StateParams {
     double H[2], S[2];                          // single barrier [state]
     double dh[2][2], ds[2][2][2], di[2][2][2];  // double barrier [state][barrier][side]
     double p_flip[2];                           // per-step flip probability [state] (EQUALPROB false)
+    uint64_t t_flip[2];                         // its flip_threshold
 };
 
 __device__ __forceinline__ double in_register(double v)
+{
+    asm("" : "+v"(v));
+    return v;
+}
+
+__device__ __forceinline__ uint64_t in_register(uint64_t v)
 {
     asm("" : "+v"(v));
     return v;
@@ -205,6 +216,8 @@ __device__ __forceinline__ StateParams state_params(const RunConst& c, const Fla
     StateParams sp;
     sp.p_flip[0] = in_register(c.p_asym0);
     sp.p_flip[1] = in_register(c.p_asym1);
+    sp.t_flip[0] = in_register(c.t_asym0);
+    sp.t_flip[1] = in_register(c.t_asym1);
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
         if (f.model == 0) {
@@ -244,12 +257,16 @@ __device__ __forceinline__ void lane_param(Lane& s, const StateParams& sp, const
         }
     }
     if (!count) return;
-    if (one) s.cnt1 += 1;
-    else s.cnt0 += 1;
+    const uint64_t inc1 = one ? 1u : 0u;
+    s.cnt1 += inc1;
+    s.cnt0 += 1u - inc1;
 }
 
 // EnergyAndForce (Single.h:1292-1324; Double.h:1464-1522)
-__device__ __forceinline__ void lane_energy_force(Lane& s, const RunConst& c, const Flags& f)
+// force: sets F; energy: sets E and barrier (the step needs F only; E and
+// barrier, functions of the same x and state, are computed when stored)
+__device__ __forceinline__ void lane_energy_force(Lane& s, const RunConst& c, const Flags& f, bool force = true,
+                                                  bool energy = true)
 {
     if (f.model == 0) {
         const double r = s.x * c.Linv;
@@ -264,9 +281,11 @@ __device__ __forceinline__ void lane_energy_force(Lane& s, const RunConst& c, co
         E = pos ? vpos : E;
         double F = neg ? slope_neg : 0.0;
         F = pos ? s.slope_pos : F;
-        s.E = E;
-        s.F = F;
-        s.barrier = s.height;
+        if (energy) {
+            s.E = E;
+            s.barrier = s.height;
+        }
+        if (force) s.F = F;
     } else {
         const double xp = s.xp;
         const bool n0 = (xp < c.xb[0]) && (xp > c.lo[0]);
@@ -290,9 +309,11 @@ __device__ __forceinline__ void lane_energy_force(Lane& s, const RunConst& c, co
         B = p0 ? s.dh[0] : B;
         B = n1 ? s.dh[1] : B;
         B = p1 ? s.dh[1] : B;
-        s.E = E;
-        s.F = F;
-        s.barrier = B;
+        if (energy) {
+            s.E = E;
+            s.barrier = B;
+        }
+        if (force) s.F = F;
     }
 }
 
@@ -347,13 +368,11 @@ __device__ __forceinline__ void replay_noise(const RunConst& c, const Flags& f, 
     u = f.tau0 ? 1.0 : P.f[(c.init_flip + k) * c.n_traj + t];
 }
 
-__device__ __forceinline__ void step_noise(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
-                                           uint64_t iStep, StepNoise& sn, double& g, double& u)
+// Philox noise of step iStep (>= 1): the Gaussian and the 32-bit word of the flip
+// uniform (unit_f64 of it)
+__device__ __forceinline__ void step_noise(const RunConst& c, uint32_t t, uint64_t iStep, StepNoise& sn, double& g,
+                                           uint32_t& fw)
 {
-    if (f.replay) {
-        replay_noise(c, f, P, t, iStep, g, u);
-        return;
-    }
     const uint64_t q = (iStep - 1) >> 1;
     if ((iStep - 1) & 1) {
         if (sn.q != q + 1) {
@@ -363,13 +382,13 @@ __device__ __forceinline__ void step_noise(const RunConst& c, const Flags& f, co
             sn.w3 = w.w[3];
         }
         g = sn.g1;
-        u = unit_f64(sn.w3);
+        fw = sn.w3;
     } else {
         const Philox4 w = philox4x32_10((uint32_t)q, (uint32_t)(q >> 32), c.traj_offset + t, 0u, c.key0, c.key1);
         box_muller_f64(w.w[0], w.w[1], g, sn.g1);
         sn.w3 = w.w[3];
         sn.q = q + 1;
-        u = unit_f64(w.w[2]);
+        fw = w.w[2];
     }
 }
 
@@ -719,12 +738,14 @@ __device__ __forceinline__ void record(const RunConst& c, const Flags& f, const 
     const uint64_t o = frame * c.n_traj + t;
     P.frames[o] = s.x;
     if (c.save_all) {
+        Lane e = s;
+        lane_energy_force(e, c, f, false, true);
         P.frames[plane * 1 + o] = f.over ? 0.0 : s.v;
         P.frames[plane * 2 + o] = s.F;
-        P.frames[plane * 3 + o] = s.E;
+        P.frames[plane * 3 + o] = e.E;
         P.frames[plane * 4 + o] = c.mhalf * (s.v * s.v);
         P.frames[plane * 5 + o] = s.state;
-        P.frames[plane * 6 + o] = s.barrier;
+        P.frames[plane * 6 + o] = e.barrier;
     }
 }
 
@@ -845,10 +866,10 @@ constexpr int kWaitVmLgkm0 = 0x0070;
 
 // One buffered noise record per trajectory-step: Gaussian (lo, hi words) and
 // the 32-bit flip word.
-__device__ __forceinline__ void buffered_noise(const uint4& r, double& g, double& u)
+__device__ __forceinline__ void buffered_noise(const uint4& r, double& g, uint32_t& fw)
 {
     g = __hiloint2double((int)r.y, (int)r.x);
-    u = unit_f64(r.z);
+    fw = r.z;
 }
 
 template <int MODEL, int OVER, int TAU0, int SRC>
@@ -861,9 +882,6 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     if (t >= c.n_traj) return;
     ldpcp_lane* const og = P.lanes + t;
     if (og->error) return;
-    // the statistics record the events update: in LDS for the buffered launch (one
-    // wavefront per workgroup, few trajectories, where each event's dependent
-    // reads are on the step's critical path), else in place
     // the buffered launch (one wavefront per workgroup, few trajectories) keeps the
     // statistics record and the barrier table in LDS: an event's reads and a
     // hand-over's table lookups are on the step's critical path
@@ -893,10 +911,66 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     // frames every save_freq steps: a countdown instead of a 64-bit modulo per step
     uint64_t to_save = c.save_freq - s0 % c.save_freq;
     uint64_t frame = s0 / c.save_freq + 1;
-    uint64_t done = s0;  // last step completed
 
-    // One step with its noise (g, u); false ends the trajectory's launch (err set).
-    auto step = [&](uint64_t iStep, double g, double u) -> bool {
+    // The step loop runs all of the launch's steps for every trajectory (uniform
+    // control flow).  A trajectory whose launch ends with an error stores its
+    // record and state at that point and is no longer live: its later steps have
+    // no effect outside registers.
+    bool live = true;
+    auto finish = [&](uint64_t last) {
+        o->n_catch = n_catch;
+        o->n_gauss += last - s0;
+        if (!f.tau0) o->n_flip += last - s0;
+        o->steps_done = last;
+        o->error = err;
+        lane_energy_force(s, c, f, false, true);
+        store_lane(s, P, t);
+    };
+    auto fail = [&](uint64_t iStep) {
+        finish(iStep - 1);
+        live = false;
+    };
+
+    // Hand-over to the barrier just entered (Single.h:2025-2068; Double.h:2466-2510): the
+    // barrier left keeps its state and the step, the barrier entered takes its stored
+    // state and catches up on the steps since it was left.  Sets err on failure.
+    auto hand_over = [&](uint64_t iStep) {
+        const int ap = f.model == 0 ? 0 : (int)s.local_prev;
+        const int ac = f.model == 0 ? 0 : (int)s.local;
+        const int kp = win_slot(w, (int)s.periodic_prev, err);
+        const int kc = win_slot(w, (int)s.periodic, err);
+        if (err) return;
+        win_put(w, ap, (int)s.periodic_prev, kp, kValid | ((int)s.state ? kState : 0ull) | (iStep & kStepMask));
+        const uint64_t ev = win_load(c, f, P, t, w, ac, (int)s.periodic, kc, err);
+        const double prev_state = (ev & kState) ? 1.0 : 0.0;
+        if (f.tau0) {
+            s.state = prev_state;
+        } else {
+            const double lap = (double)(iStep - (ev & kStepMask));
+            double pc;
+            if (c.equal_prob) pc = 0.5 * (1.0 - exp(c.c_lap * lap));
+            else if (prev_state == 0.0) pc = c.K / (1.0 + c.K) * (1.0 - exp(c.c_lap * lap));
+            else pc = 1.0 / (1.0 + c.K) * (1.0 - exp(c.c_lap * lap));
+            double uc;
+            if (f.replay) {
+                if (n_catch >= c.catch_cap) {
+                    err = 4;
+                    return;
+                }
+                uc = P.c[(uint64_t)t * c.catch_cap + n_catch];
+            } else {
+                const Philox4 w = philox4x32_10((uint32_t)n_catch, (uint32_t)(n_catch >> 32), c.traj_offset + t,
+                                                2u, c.key0, c.key1);
+                uc = unit_f64(w.w[0]);
+            }
+            n_catch++;
+            s.state = (uc < pc) ? ((prev_state == 1.0) ? 0.0 : 1.0) : prev_state;
+        }
+    };
+
+    // One step with its noise: the Gaussian g and the flip uniform uw, either as
+    // its 32-bit Philox word (tested against flip_threshold) or, replayed, as a double.
+    auto step = [&](uint64_t iStep, double g, auto uw) {
         if (f.over) {
             const double rf = c.var * g;
             s.x = fma(s.F, c.dtgm, s.x);
@@ -910,64 +984,34 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
             s.x = s.x + s.v * c.halfdt;
         }
         lane_indices(s, c, f, false);
-        // per-step flip of the barrier under the particle (Single.h:2010-2015, 2078-2086)
+        // per-step flip of the barrier under the particle (Single.h:2010-2015, 2078-2086);
+        // the state is 0.0 or 1.0, so a flip is 1 - state
         if (!f.tau0) {
-            if (c.equal_prob) {
-                if (u < c.p_flip) s.state = (s.state == 0.0) ? 1.0 : 0.0;
-            } else {
-                const bool one = (s.state == 1.0);
-                if (u < (one ? sp.p_flip[1] : sp.p_flip[0])) s.state = one ? 0.0 : 1.0;
-            }
+            const bool one = (s.state == 1.0);
+            bool flip;
+            if constexpr (sizeof(uw) == sizeof(uint32_t))
+                flip = (uint64_t)uw < (c.equal_prob ? c.t_flip : (one ? sp.t_flip[1] : sp.t_flip[0]));
+            else
+                flip = uw < (c.equal_prob ? c.p_flip : (one ? sp.p_flip[1] : sp.p_flip[0]));
+            s.state = flip ? 1.0 - s.state : s.state;
         }
-        // hand-over to the barrier just entered (Single.h:2025-2068; Double.h:2466-2510)
         const bool entering = f.model == 0 ? s.ent_period : s.ent_local;
-        if (entering) {
-            const int ap = f.model == 0 ? 0 : (int)s.local_prev;
-            const int ac = f.model == 0 ? 0 : (int)s.local;
-            const int kp = win_slot(w, (int)s.periodic_prev, err);
-            const int kc = win_slot(w, (int)s.periodic, err);
-            if (err) return false;
-            win_put(w, ap, (int)s.periodic_prev, kp, kValid | ((int)s.state ? kState : 0ull) | (iStep & kStepMask));
-            const uint64_t ev = win_load(c, f, P, t, w, ac, (int)s.periodic, kc, err);
-            const double prev_state = (ev & kState) ? 1.0 : 0.0;
-            if (f.tau0) {
-                s.state = prev_state;
-            } else {
-                const double lap = (double)(iStep - (ev & kStepMask));
-                double pc;
-                if (c.equal_prob) pc = 0.5 * (1.0 - exp(c.c_lap * lap));
-                else if (prev_state == 0.0) pc = c.K / (1.0 + c.K) * (1.0 - exp(c.c_lap * lap));
-                else pc = 1.0 / (1.0 + c.K) * (1.0 - exp(c.c_lap * lap));
-                double uc;
-                if (f.replay) {
-                    if (n_catch >= c.catch_cap) {
-                        err = 4;
-                        return false;
-                    }
-                    uc = P.c[(uint64_t)t * c.catch_cap + n_catch];
-                } else {
-                    const Philox4 w = philox4x32_10((uint32_t)n_catch, (uint32_t)(n_catch >> 32), c.traj_offset + t,
-                                                    2u, c.key0, c.key1);
-                    uc = unit_f64(w.w[0]);
-                }
-                n_catch++;
-                s.state = (uc < pc) ? ((prev_state == 1.0) ? 0.0 : 1.0) : prev_state;
-            }
-            if (err) return false;
+        if (entering && live) {
+            hand_over(iStep);
+            if (err) fail(iStep);
         }
         lane_param(s, sp, f);
-        lane_energy_force(s, c, f);
+        lane_energy_force(s, c, f, true, false);
         if (!f.over) s.v = s.v + (c.minv * s.F) * c.halfdt;
-        if (s.ent_period || s.ent_well) {
+        if ((s.ent_period || s.ent_well) && live) {
             stats_event(c, f, P, t, w, o, s, iStep, P.ticks + (uint64_t)t * c.tick_cap, err);
-            if (err) return false;
+            if (err) fail(iStep);
         }
         if (--to_save == 0) {
-            record(c, f, P, t, s, frame++);
+            if (live) record(c, f, P, t, s, frame);
+            frame++;
             to_save = c.save_freq;
         }
-        done = iStep;
-        return true;
     };
 
     const uint64_t len = s1 - s0;
@@ -990,19 +1034,19 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         };
         // steps of stage q from buf; each step reads the next step's record, so
         // the LDS latency overlaps a step
-        auto run_stage = [&](uint4 (&buf)[kStage][64], uint64_t q) -> bool {
+        auto run_stage = [&](uint4 (&buf)[kStage][64], uint64_t q) {
             const uint64_t first = q * kStage;
             const uint32_t m = (uint32_t)(len - first < kStage ? len - first : kStage);
             uint4 rec = buf[0][lane];
             for (uint32_t j = 0; j < m; ++j) {
                 uint4 next = rec;
                 if (j + 1 < m) next = buf[j + 1][lane];
-                double g, u;
-                buffered_noise(rec, g, u);
-                if (!step(s0 + first + j + 1, g, u)) return false;
+                double g;
+                uint32_t fw;
+                buffered_noise(rec, g, fw);
+                step(s0 + first + j + 1, g, fw);
                 rec = next;
             }
-            return true;
         };
         // stage q has landed (and every earlier LDS read is done): load stage q + 1
         // into the other array, whose stage q - 1 has been read
@@ -1014,11 +1058,11 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         for (uint64_t q = 0; q * kStage < len; q += 2) {
             boundary();
             load_stage(stage_b, q + 1);
-            if (!run_stage(stage_a, q)) break;
+            run_stage(stage_a, q);
             if ((q + 1) * kStage >= len) break;
             boundary();
             load_stage(stage_a, q + 2);
-            if (!run_stage(stage_b, q + 1)) break;
+            run_stage(stage_b, q + 1);
         }
     } else if constexpr (REPLAY) {
         // the noise of the next kAhead steps is in flight while a step runs
@@ -1036,7 +1080,7 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
                 uq[d] = uq[d + 1];
             }
             replay_noise(c, f, P, t, (iStep + kAhead < s1) ? iStep + kAhead : s1, gq[kAhead - 1], uq[kAhead - 1]);
-            if (!step(iStep, g, u)) break;
+            step(iStep, g, u);
         }
     } else {
         StepNoise sn;
@@ -1044,21 +1088,17 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         sn.g1 = 0.0;
         sn.w3 = 0;
         for (uint64_t iStep = s0 + 1; iStep <= s1; ++iStep) {
-            double g, u;
-            step_noise(c, f, P, t, iStep, sn, g, u);
-            if (!step(iStep, g, u)) break;
+            double g;
+            uint32_t fw;
+            step_noise(c, t, iStep, sn, g, fw);
+            step(iStep, g, fw);
         }
     }
-    o->n_catch = n_catch;
-    o->n_gauss += done - s0;
-    if (!f.tau0) o->n_flip += done - s0;
-    o->steps_done = done;
-    o->error = err;
+    if (live) finish(s1);
     if constexpr (kLocal) {
         *og = *o;
         win_flush(w);
     }
-    store_lane(s, P, t);
 }
 
 // MeanSquareDisplacementBruteForce (Single.h:24-45) per (trajectory, lag)
