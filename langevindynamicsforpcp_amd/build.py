@@ -73,7 +73,7 @@ def build_executables():
         if not os.path.exists(main_src):
             continue
         exe = os.path.join(BIN_DIR, name)
-        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"), "-I", HOST,
+        _run(["g++", "-O2", "-std=c++17", "-pthread", "-Wall", "-I", os.path.join(REPO, "include"), "-I", HOST,
               main_src, *common, "-o", exe, f"-L{LIB_DIR}", "-lldpcp", "-Wl,-rpath,$ORIGIN/../lib"])
         out.append(exe)
     return out

@@ -14,6 +14,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "ldpcp_common.hpp"
@@ -1126,11 +1127,13 @@ __global__ void __launch_bounds__(256) msd_kernel(const double* __restrict__ fra
 // [step][trajectory] as buffered_noise reads it: the values step_noise
 // would draw in the loop (Gaussian of the step's Box-Muller pair half, flip
 // word w[2] / w[3]).
-__global__ void __launch_bounds__(256) run_noise_kernel(RunConst c, uint64_t s0, uint4* __restrict__ noise)
+__global__ void __launch_bounds__(256) run_noise_kernel(RunConst c, uint64_t s0, uint64_t len,
+                                                        uint4* __restrict__ noise)
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= c.n_traj) return;
-    const uint64_t k = blockIdx.y;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // k * n_traj + t
+    if (i >= len * c.n_traj) return;
+    const uint64_t k = i / c.n_traj;
+    const uint32_t t = (uint32_t)(i - k * c.n_traj);
     const uint64_t iStep = s0 + 1 + k;
     const uint64_t q = (iStep - 1) >> 1;
     const Philox4 w = philox4x32_10((uint32_t)q, (uint32_t)(q >> 32), c.traj_offset + t, 0u, c.key0, c.key1);
@@ -1196,14 +1199,19 @@ StepKernel step_kernel_for(const RunConst& c, int src)
 // Handles with at most this many trajectories fill a small part of the GPU, so
 // each step's latency decides their time: they take the buffered noise.
 constexpr uint32_t kBufferedMaxTraj = 8192;
-constexpr uint64_t kBufferedLaunch = 8192;       // most steps per launch (and per noise buffer)
-constexpr uint64_t kNoiseBufferBytes = 1ull << 28;  // noise buffer bound: fewer steps per launch above 2048 trajectories
+constexpr uint64_t kBufferedLaunch = 1ull << 20;  // most steps per launch (and per noise buffer)
+constexpr uint64_t kNoiseBufferBytes = 1ull << 28;  // noise buffer bound: fewer steps per launch above 16 trajectories
 
-// Steps per launch of a buffered-noise handle of n trajectories.
-uint64_t buffered_steps(uint64_t n)
+// Steps per launch of a buffered-noise handle of n trajectories whose runs are
+// nsteps long.  Long launches keep the number of kernels a run enqueues small:
+// a stream's hardware queue holds a bounded number of them, and a launch call
+// that fills it waits, which would hold back the launch of the next handle.
+uint64_t buffered_steps(uint64_t n, uint64_t nsteps)
 {
-    const uint64_t fit = kNoiseBufferBytes / (n * sizeof(uint4));
-    return fit < kBufferedLaunch ? fit : kBufferedLaunch;
+    uint64_t k = kNoiseBufferBytes / (n * sizeof(uint4));
+    if (k > kBufferedLaunch) k = kBufferedLaunch;
+    if (k > nsteps) k = nsteps;
+    return k ? k : 1;
 }
 
 }  // namespace
@@ -1217,7 +1225,7 @@ struct ldpcp_run {
     hipStream_t stream;
     void* state_block;
     void* frames_block;   // frames of a resumed run (prior frames + this run's), else null
-    uint4* noise;         // buffered noise, [buffered_steps(n_traj)][n_traj], for handles of few trajectories
+    uint4* noise;         // buffered noise, [buffered_steps(n_traj, nsteps)][n_traj], for handles of few trajectories
     bool events_complete; // the event lists hold every event since step 0
     uint64_t n_frames;
     uint64_t step_base;   // absolute step the next integrate starts after
@@ -1329,7 +1337,7 @@ extern "C" int ldpcp_run_create(const ldpcp_run_params* p, uint32_t n_traj, uint
     }
     // the buffered-noise scratch and the event lists are allocated here, not at
     // launch: an allocation between launches could wait for other handles' runs
-    if (n <= kBufferedMaxTraj) e = hipMalloc(&h->noise, buffered_steps(n) * n * sizeof(uint4));
+    if (n <= kBufferedMaxTraj) e = hipMalloc(&h->noise, buffered_steps(n, h->c.nsteps_eff) * n * sizeof(uint4));
     if (e == hipSuccess && h->c.n_lists)
         e = hipMalloc(&h->P.events, n * (uint64_t)h->c.n_lists * h->c.ev_cap * sizeof(double));
     if (e != hipSuccess) {
@@ -1446,15 +1454,13 @@ extern "C" int ldpcp_run_launch(ldpcp_run* h, uint64_t steps_per_launch)
     const bool buffered = !h->c.replay && n <= kBufferedMaxTraj;
     const int src = h->c.replay ? kNoiseReplay : buffered ? kNoiseBuffered : kNoisePhilox;
     const StepKernel step = step_kernel_for(h->c, src);
-    if (buffered) {
-        if (steps_per_launch > buffered_steps(n)) steps_per_launch = buffered_steps(n);
-    }
+    if (buffered) steps_per_launch = std::min(steps_per_launch, buffered_steps(n, h->c.nsteps_eff));
     const dim3 sblock(buffered ? 64 : 256), sgrid((unsigned)((n + sblock.x - 1) / sblock.x));
     for (uint64_t s0 = h->step_base; s0 < end; s0 += steps_per_launch) {
         const uint64_t s1 = (s0 + steps_per_launch < end) ? s0 + steps_per_launch : end;
         if (buffered) {
-            const dim3 ngrid((unsigned)((n + 255) / 256), (unsigned)(s1 - s0));
-            hipLaunchKernelGGL(run_noise_kernel, ngrid, dim3(256), 0, h->stream, h->c, s0, h->noise);
+            const dim3 ngrid((unsigned)(((s1 - s0) * n + 255) / 256));
+            hipLaunchKernelGGL(run_noise_kernel, ngrid, dim3(256), 0, h->stream, h->c, s0, s1 - s0, h->noise);
             LDPCP_HIP_TRY(hipGetLastError());
         }
         step<<<sgrid, sblock, 0, h->stream>>>(h->c, h->P, s0, s1, h->noise);

@@ -12,6 +12,7 @@
 #include <string.h>
 #include <chrono>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ldpcp.h"
@@ -410,9 +411,26 @@ int run(const RunnerConfig& cfg)
                         exit(0);
                     }
                 }
-                if (ldpcp_run_launch(h, cfg.steps_per_launch)) die("ldpcp_run_launch");
                 handles[(size_t)(i - i0) * R + r] = h;
                 before[(size_t)(i - i0) * R + r] = nsteps_before;
+            }
+        // Each handle is launched from a thread of its own: a launch enqueues its
+        // run's kernels on the handle's stream and waits while the stream's hardware
+        // queue is full, so launches made one after another from one thread would
+        // run a long trial's ranks one after another.
+        std::vector<std::string> launch_error(handles.size());
+        {
+            std::vector<std::thread> launchers;
+            for (size_t k = 0; k < handles.size(); ++k)
+                launchers.emplace_back([&, k] {
+                    if (ldpcp_run_launch(handles[k], cfg.steps_per_launch)) launch_error[k] = ldpcp_last_error();
+                });
+            for (std::thread& th : launchers) th.join();
+        }
+        for (const std::string& msg : launch_error)
+            if (!msg.empty()) {
+                fprintf(stderr, "Error: ldpcp_run_launch: %s\n", msg.c_str());
+                exit(1);
             }
         for (ldpcp_run* h : handles)
             if (ldpcp_run_wait(h)) die("ldpcp_run_integrate");
