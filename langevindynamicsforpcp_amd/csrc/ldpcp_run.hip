@@ -64,6 +64,13 @@ struct RunPtrs {
     double* events;      // event lists [traj][list][ev_cap]
 };
 
+// The kernel arguments as one block in device memory (a handle's copy, written at
+// each launch), for the step kernel's out-of-line branches.
+struct RunArgs {
+    RunConst c;
+    RunPtrs P;
+};
+
 // The run's branch flags, separate from RunConst so that a kernel can make them
 // compile-time constants while the constants stay in the kernel arguments.
 struct Flags {
@@ -846,6 +853,98 @@ __global__ void __launch_bounds__(256) run_init_kernel(RunConst c, RunPtrs P)
     store_lane(s, P, t);
 }
 
+// The step's rare branches, a hand-over and a statistics event, as functions of
+// their own: called on the few steps that need them (a hand-over or an event on
+// any trajectory of the wavefront: ~3 % of the steps of the quick test's
+// overdamped 16 trajectories), they keep their registers and branches out of the
+// step loop.  They read the run's constants and pointers from `args`.
+
+struct HandOver {
+    double state;
+    uint64_t n_catch;
+    int err;
+};
+
+// `args` is the same for every lane: as a scalar address into constant memory its
+// fields are read with scalar loads
+using ConstArgs = const __attribute__((address_space(4))) RunArgs*;
+__device__ __forceinline__ ConstArgs uniform_args(const RunArgs* p)
+{
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (ConstArgs)(((uint64_t)hi << 32) | lo);
+}
+
+// Hand-over to the barrier just entered (Single.h:2025-2068; Double.h:2466-2510):
+// the barrier left keeps its state and the step, the barrier entered takes its
+// stored state and catches up on the steps since it was left.
+template <int MODEL, int OVER, int TAU0, int REPLAY>
+__device__ __noinline__ HandOver hand_over_call(const RunArgs* args, uint32_t t, Window w, uint64_t iStep,
+                                                double periodic_prev, double periodic, double local_prev,
+                                                double local, double state, uint64_t n_catch)
+{
+    constexpr Flags f{MODEL, OVER, TAU0, REPLAY};
+    const ConstArgs a = uniform_args(args);
+    const RunConst& c = (const RunConst&)a->c;
+    const RunPtrs& P = (const RunPtrs&)a->P;
+    HandOver r{state, n_catch, 0};
+    const int ap = f.model == 0 ? 0 : (int)local_prev;
+    const int ac = f.model == 0 ? 0 : (int)local;
+    const int kp = win_slot(w, (int)periodic_prev, r.err);
+    const int kc = win_slot(w, (int)periodic, r.err);
+    if (r.err) return r;
+    win_put(w, ap, (int)periodic_prev, kp, kValid | ((int)state ? kState : 0ull) | (iStep & kStepMask));
+    const uint64_t ev = win_load(c, f, P, t, w, ac, (int)periodic, kc, r.err);
+    const double prev_state = (ev & kState) ? 1.0 : 0.0;
+    if (f.tau0) {
+        r.state = prev_state;
+        return r;
+    }
+    const double lap = (double)(iStep - (ev & kStepMask));
+    double pc;
+    if (c.equal_prob) pc = 0.5 * (1.0 - exp(c.c_lap * lap));
+    else if (prev_state == 0.0) pc = c.K / (1.0 + c.K) * (1.0 - exp(c.c_lap * lap));
+    else pc = 1.0 / (1.0 + c.K) * (1.0 - exp(c.c_lap * lap));
+    double uc;
+    if (f.replay) {
+        if (n_catch >= c.catch_cap) {
+            r.err = 4;
+            return r;
+        }
+        uc = P.c[(uint64_t)t * c.catch_cap + n_catch];
+    } else {
+        const Philox4 u4 = philox4x32_10((uint32_t)n_catch, (uint32_t)(n_catch >> 32), c.traj_offset + t, 2u, c.key0,
+                                         c.key1);
+        uc = unit_f64(u4.w[0]);
+    }
+    r.n_catch = n_catch + 1;
+    r.state = (uc < pc) ? ((prev_state == 1.0) ? 0.0 : 1.0) : prev_state;
+    return r;
+}
+
+// stats_event of a step with ent_well or ent_period; returns its error code
+template <int MODEL, int OVER, int TAU0, int REPLAY>
+__device__ __noinline__ int stats_event_call(const RunArgs* args, uint32_t t, Window w, ldpcp_lane* o, uint64_t iStep,
+                                             double well, double periodic, double state, bool ent_well,
+                                             bool ent_period)
+{
+    constexpr Flags f{MODEL, OVER, TAU0, REPLAY};
+    const ConstArgs a = uniform_args(args);
+    const RunConst& c = (const RunConst&)a->c;
+    const RunPtrs& P = (const RunPtrs&)a->P;
+    Lane s;  // the fields the statistics read
+    s.well = well;
+    s.periodic = periodic;
+    s.state = state;
+    s.ent_well = ent_well;
+    s.ent_period = ent_period;
+    s.wbase = w.wbase;
+    int err = 0;
+    stats_event(c, f, P, t, w, o, s, iStep, P.ticks + (uint64_t)t * c.tick_cap, err);
+    return err;
+}
+
 // Steps (s0, s1] of IntegrateStepEta / IntegrateStep
 // (underdamped Single.h:1654-1706; overdamped :2833-2886; Double.h:1971-2024, 3475-3508).
 // One instantiation per (model, integrator, TAU = 0, noise source): the flags
@@ -875,7 +974,7 @@ __device__ __forceinline__ void buffered_noise(const uint4& r, double& g, uint32
 
 template <int MODEL, int OVER, int TAU0, int SRC>
 __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, uint64_t s0, uint64_t s1,
-                                                       const uint4* __restrict__ noise)
+                                                       const uint4* __restrict__ noise, const RunArgs* args)
 {
     constexpr int REPLAY = (SRC == kNoiseReplay);
     constexpr Flags f{MODEL, OVER, TAU0, REPLAY};
@@ -932,43 +1031,6 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         live = false;
     };
 
-    // Hand-over to the barrier just entered (Single.h:2025-2068; Double.h:2466-2510): the
-    // barrier left keeps its state and the step, the barrier entered takes its stored
-    // state and catches up on the steps since it was left.  Sets err on failure.
-    auto hand_over = [&](uint64_t iStep) {
-        const int ap = f.model == 0 ? 0 : (int)s.local_prev;
-        const int ac = f.model == 0 ? 0 : (int)s.local;
-        const int kp = win_slot(w, (int)s.periodic_prev, err);
-        const int kc = win_slot(w, (int)s.periodic, err);
-        if (err) return;
-        win_put(w, ap, (int)s.periodic_prev, kp, kValid | ((int)s.state ? kState : 0ull) | (iStep & kStepMask));
-        const uint64_t ev = win_load(c, f, P, t, w, ac, (int)s.periodic, kc, err);
-        const double prev_state = (ev & kState) ? 1.0 : 0.0;
-        if (f.tau0) {
-            s.state = prev_state;
-        } else {
-            const double lap = (double)(iStep - (ev & kStepMask));
-            double pc;
-            if (c.equal_prob) pc = 0.5 * (1.0 - exp(c.c_lap * lap));
-            else if (prev_state == 0.0) pc = c.K / (1.0 + c.K) * (1.0 - exp(c.c_lap * lap));
-            else pc = 1.0 / (1.0 + c.K) * (1.0 - exp(c.c_lap * lap));
-            double uc;
-            if (f.replay) {
-                if (n_catch >= c.catch_cap) {
-                    err = 4;
-                    return;
-                }
-                uc = P.c[(uint64_t)t * c.catch_cap + n_catch];
-            } else {
-                const Philox4 w = philox4x32_10((uint32_t)n_catch, (uint32_t)(n_catch >> 32), c.traj_offset + t,
-                                                2u, c.key0, c.key1);
-                uc = unit_f64(w.w[0]);
-            }
-            n_catch++;
-            s.state = (uc < pc) ? ((prev_state == 1.0) ? 0.0 : 1.0) : prev_state;
-        }
-    };
-
     // One step with its noise: the Gaussian g and the flip uniform uw, either as
     // its 32-bit Philox word (tested against flip_threshold) or, replayed, as a double.
     auto step = [&](uint64_t iStep, double g, auto uw) {
@@ -998,14 +1060,19 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         }
         const bool entering = f.model == 0 ? s.ent_period : s.ent_local;
         if (entering && live) {
-            hand_over(iStep);
+            const HandOver r = hand_over_call<MODEL, OVER, TAU0, REPLAY>(
+                args, t, w, iStep, s.periodic_prev, s.periodic, s.local_prev, s.local, s.state, n_catch);
+            s.state = r.state;
+            n_catch = r.n_catch;
+            err = r.err;
             if (err) fail(iStep);
         }
         lane_param(s, sp, f);
         lane_energy_force(s, c, f, true, false);
         if (!f.over) s.v = s.v + (c.minv * s.F) * c.halfdt;
         if ((s.ent_period || s.ent_well) && live) {
-            stats_event(c, f, P, t, w, o, s, iStep, P.ticks + (uint64_t)t * c.tick_cap, err);
+            err = stats_event_call<MODEL, OVER, TAU0, REPLAY>(args, t, w, o, iStep, s.well, s.periodic, s.state,
+                                                              s.ent_well, s.ent_period);
             if (err) fail(iStep);
         }
         if (--to_save == 0) {
@@ -1174,7 +1241,7 @@ __global__ void __launch_bounds__(256) run_table_kernel(RunConst c, RunPtrs P, i
     steps[(uint64_t)t * n_ids + k] = step;
 }
 
-using StepKernel = void (*)(RunConst, RunPtrs, uint64_t, uint64_t, const uint4*);
+using StepKernel = void (*)(RunConst, RunPtrs, uint64_t, uint64_t, const uint4*, const RunArgs*);
 
 template <int MODEL, int OVER, int SRC>
 StepKernel step_kernel_tau(const RunConst& c)
@@ -1226,6 +1293,8 @@ struct ldpcp_run {
     void* state_block;
     void* frames_block;   // frames of a resumed run (prior frames + this run's), else null
     uint4* noise;         // buffered noise, [buffered_steps(n_traj, nsteps)][n_traj], for handles of few trajectories
+    ldpcp::RunArgs* args; // device copy of c and P for the step kernel, written at each launch
+    ldpcp::RunArgs args_host;
     bool events_complete; // the event lists hold every event since step 0
     uint64_t n_frames;
     uint64_t step_base;   // absolute step the next integrate starts after
@@ -1338,11 +1407,13 @@ extern "C" int ldpcp_run_create(const ldpcp_run_params* p, uint32_t n_traj, uint
     // the buffered-noise scratch and the event lists are allocated here, not at
     // launch: an allocation between launches could wait for other handles' runs
     if (n <= kBufferedMaxTraj) e = hipMalloc(&h->noise, buffered_steps(n, h->c.nsteps_eff) * n * sizeof(uint4));
+    if (e == hipSuccess) e = hipMalloc(&h->args, sizeof(RunArgs));
     if (e == hipSuccess && h->c.n_lists)
         e = hipMalloc(&h->P.events, n * (uint64_t)h->c.n_lists * h->c.ev_cap * sizeof(double));
     if (e != hipSuccess) {
         set_error("ldpcp_run_create: hipMalloc of the noise buffer or event lists failed: %s", hipGetErrorString(e));
         if (h->noise) (void)hipFree(h->noise);
+        if (h->args) (void)hipFree(h->args);
         (void)hipStreamDestroy(h->stream);
         (void)hipFree(h->state_block);
         delete h;
@@ -1456,6 +1527,11 @@ extern "C" int ldpcp_run_launch(ldpcp_run* h, uint64_t steps_per_launch)
     const StepKernel step = step_kernel_for(h->c, src);
     if (buffered) steps_per_launch = std::min(steps_per_launch, buffered_steps(n, h->c.nsteps_eff));
     const dim3 sblock(buffered ? 64 : 256), sgrid((unsigned)((n + sblock.x - 1) / sblock.x));
+    // the step kernel's copy of its arguments (the handle keeps the host side
+    // unchanged while the run is pending)
+    h->args_host.c = h->c;
+    h->args_host.P = h->P;
+    LDPCP_HIP_TRY(hipMemcpyAsync(h->args, &h->args_host, sizeof(RunArgs), hipMemcpyHostToDevice, h->stream));
     for (uint64_t s0 = h->step_base; s0 < end; s0 += steps_per_launch) {
         const uint64_t s1 = (s0 + steps_per_launch < end) ? s0 + steps_per_launch : end;
         if (buffered) {
@@ -1463,7 +1539,7 @@ extern "C" int ldpcp_run_launch(ldpcp_run* h, uint64_t steps_per_launch)
             hipLaunchKernelGGL(run_noise_kernel, ngrid, dim3(256), 0, h->stream, h->c, s0, s1 - s0, h->noise);
             LDPCP_HIP_TRY(hipGetLastError());
         }
-        step<<<sgrid, sblock, 0, h->stream>>>(h->c, h->P, s0, s1, h->noise);
+        step<<<sgrid, sblock, 0, h->stream>>>(h->c, h->P, s0, s1, h->noise, h->args);
         LDPCP_HIP_TRY(hipGetLastError());
     }
     h->pending = true;
@@ -1955,6 +2031,7 @@ extern "C" int ldpcp_run_destroy(ldpcp_run* h)
     if (h->c.replay) (void)hipFree((void*)h->P.g);
     if (h->frames_block) (void)hipFree(h->frames_block);
     if (h->noise) (void)hipFree(h->noise);
+    if (h->args) (void)hipFree(h->args);
     if (h->P.events) (void)hipFree(h->P.events);
     (void)hipFree(h->state_block);
     (void)hipStreamDestroy(h->stream);
