@@ -33,6 +33,7 @@ struct RunConst {
     int32_t model, overdamped, tau_zero, equal_prob, same_init, centre;
     int32_t b_wait, b_fpt, b_tpt, b_cg, n_cg, save_all, randomize_v, randomize_state, replay, wb;
     int32_t cg_size[LDPCP_MAX_CG];
+    double cg_inv[LDPCP_MAX_CG];  // 1 / cg_size
     uint64_t t_flip, t_asym0, t_asym1;  // flip_threshold of p_flip, p_asym0, p_asym1
     double dt, minv, mhalf, halfdt, expmgdt, var, dtgm, L, Linv, c_lap, p_flip, p_asym0, p_asym1, K, p0,
         p_create;
@@ -94,7 +95,10 @@ void make_const(const ldpcp_run_params& p, RunConst& c)
     c.b_tpt = p.b_tpt;
     c.b_cg = p.b_fptcg;
     c.n_cg = p.n_cg;
-    for (int j = 0; j < p.n_cg && j < LDPCP_MAX_CG; ++j) c.cg_size[j] = p.cg_size[j];
+    for (int j = 0; j < p.n_cg && j < LDPCP_MAX_CG; ++j) {
+        c.cg_size[j] = p.cg_size[j];
+        c.cg_inv[j] = 1.0 / p.cg_size[j];
+    }
     c.save_all = p.save_all;
     c.randomize_v = p.randomize_v;
     c.randomize_state = p.randomize_state;
@@ -555,20 +559,30 @@ __device__ __forceinline__ int stored_state(const RunConst& c, const Flags& f, c
 // side of the well the particle sits in, the one under the particle with its
 // current state, the other with its stored state.
 __device__ __forceinline__ void fpt1_arm(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t t,
-                                         const Window& w, ldpcp_lane* o, const Lane& s, int& err)
+                                         const Window& w, int32_t (&init_index)[2], double (&init_state)[2],
+                                         const Lane& s, int& err)
 {
     const int pidx = (int)s.periodic;
     if (s.well == s.periodic) {
-        o->fpt_init_state[0] = s.state;
-        o->fpt_init_index[0] = pidx;
-        o->fpt_init_state[1] = (double)stored_state(c, f, P, t, w, pidx + 1, err);
-        o->fpt_init_index[1] = pidx + 1;
+        init_state[0] = s.state;
+        init_index[0] = pidx;
+        init_state[1] = (double)stored_state(c, f, P, t, w, pidx + 1, err);
+        init_index[1] = pidx + 1;
     } else {
-        o->fpt_init_state[0] = (double)stored_state(c, f, P, t, w, pidx - 1, err);
-        o->fpt_init_index[0] = pidx - 1;
-        o->fpt_init_state[1] = s.state;
-        o->fpt_init_index[1] = pidx;
+        init_state[0] = (double)stored_state(c, f, P, t, w, pidx - 1, err);
+        init_index[0] = pidx - 1;
+        init_state[1] = s.state;
+        init_index[1] = pidx;
     }
+}
+
+// Whether FPTCG size j divides the well index (an integer-valued double below
+// 2^31 in magnitude): rint(well / size) computed with the reciprocal is exact
+// when size divides well, and k * size is a multiple of size otherwise, so the
+// test equals (int)well % size == 0 without an integer division.
+__device__ __forceinline__ bool cg_divides(const RunConst& c, int j, double well)
+{
+    return rint(well * c.cg_inv[j]) * (double)c.cg_size[j] == well;
 }
 
 // Appends v as entry n of list `list` when the lists are kept.
@@ -587,7 +601,7 @@ __device__ __forceinline__ void stats_init(const RunConst& c, const Flags& f, co
     if (c.b_fpt) {
         o->fpt_last_tick = 0.0;
         o->fpt_to_update = c.centre ? 0 : 1;
-        if (f.model == 0 && c.centre) fpt1_arm(c, f, P, t, w, o, s, err);
+        if (f.model == 0 && c.centre) fpt1_arm(c, f, P, t, w, o->fpt_init_index, o->fpt_init_state, s, err);
         o->fpt2_last_tick = 0.0;
         if (c.centre) {
             o->fpt2_to_update = 0;
@@ -610,7 +624,7 @@ __device__ __forceinline__ void stats_init(const RunConst& c, const Flags& f, co
         for (int j = 0; j < c.n_cg; ++j) {
             o->cg_last_tick[j] = 0.0;
             o->cg_initial_well[j] = (int)s.well;
-            o->cg_to_update[j] = !(c.centre && ((int)rint(s.well) % c.cg_size[j] == 0));
+            o->cg_to_update[j] = !(c.centre && cg_divides(c, j, s.well));
         }
     }
 }
@@ -624,6 +638,54 @@ __device__ __forceinline__ double duration(const RunConst& c, uint64_t iStep, do
     return fma(c.dt, (double)iStep, -last);
 }
 
+// The statistics state an event reads and updates (the FPTCG lists aside). An
+// event reads it from the lane record in one batch of independent loads and
+// writes it back at its end, so that it waits for the record once instead of
+// once per branch.
+struct EventState {
+    double wt_tick_prev, wt_last, wt_sum, wt_sum2, wt_max;
+    uint64_t wt_ticks, wt_n;
+    int32_t wt_passed;
+    double fpt_last_tick, fpt_sum;
+    uint64_t fpt_n;
+    int32_t fpt_to_update;
+    int32_t fpt_init_index[2];
+    double fpt_init_state[2];
+    double fpt2_last_tick, fpt2_sum;
+    uint64_t fpt2_n;
+    int32_t fpt2_to_update;
+    int32_t fpt2_idx[2];
+    double tpt_last_tick, tpt_sum;
+    uint64_t tpt_n;
+    int32_t tpt_to_update;
+    double tpt2_last_tick, tpt2_sum;
+    uint64_t tpt2_n;
+    int32_t tpt2_to_update, tpt2_initial_well;
+};
+
+#define LDPCP_EVENT_FIELDS(X)                                                                                  \
+    X(wt_tick_prev) X(wt_last) X(wt_sum) X(wt_sum2) X(wt_max) X(wt_ticks) X(wt_n) X(wt_passed)                \
+    X(fpt_last_tick) X(fpt_sum) X(fpt_n) X(fpt_to_update) X(fpt_init_index[0]) X(fpt_init_index[1])           \
+    X(fpt_init_state[0]) X(fpt_init_state[1]) X(fpt2_last_tick) X(fpt2_sum) X(fpt2_n) X(fpt2_to_update)       \
+    X(fpt2_idx[0]) X(fpt2_idx[1]) X(tpt_last_tick) X(tpt_sum) X(tpt_n) X(tpt_to_update) X(tpt2_last_tick)     \
+    X(tpt2_sum) X(tpt2_n) X(tpt2_to_update) X(tpt2_initial_well)
+
+__device__ __forceinline__ EventState load_event_state(const ldpcp_lane* o)
+{
+    EventState e;
+#define LDPCP_LOAD(m) e.m = o->m;
+    LDPCP_EVENT_FIELDS(LDPCP_LOAD)
+#undef LDPCP_LOAD
+    return e;
+}
+
+__device__ __forceinline__ void store_event_state(ldpcp_lane* o, const EventState& e)
+{
+#define LDPCP_STORE(m) o->m = e.m;
+    LDPCP_EVENT_FIELDS(LDPCP_STORE)
+#undef LDPCP_STORE
+}
+
 __device__ __forceinline__ void stats_event(const RunConst& c, const Flags& f, const RunPtrs& P, uint32_t traj,
                                             const Window& w, ldpcp_lane* o, const Lane& s, uint64_t iStep,
                                             double* ticks, int& err)
@@ -631,96 +693,97 @@ __device__ __forceinline__ void stats_event(const RunConst& c, const Flags& f, c
     const double t = c.dt * (double)iStep;
     double* const ev = c.ev_cap ? P.events + (uint64_t)traj * c.n_lists * c.ev_cap : nullptr;
     const bool single = (f.model == 0);
+    EventState e = load_event_state(o);
     if (c.b_wait && single) {
         if (s.ent_well) {
-            o->wt_tick_prev = (double)iStep;
-            o->wt_passed = 1;
-        } else if (s.ent_period && o->wt_passed) {
-            const double tick = o->wt_tick_prev;
-            if (o->wt_ticks < c.tick_cap) ticks[o->wt_ticks] = tick;
-            if (o->wt_ticks > 0) {
-                const double w = (tick - o->wt_last) * c.dt;
-                o->wt_sum += w;
-                o->wt_sum2 = fma(w, w, o->wt_sum2);
-                if (o->wt_n == 0 || w > o->wt_max) o->wt_max = w;
-                o->wt_n++;
+            e.wt_tick_prev = (double)iStep;
+            e.wt_passed = 1;
+        } else if (s.ent_period && e.wt_passed) {
+            const double tick = e.wt_tick_prev;
+            if (e.wt_ticks < c.tick_cap) ticks[e.wt_ticks] = tick;
+            if (e.wt_ticks > 0) {
+                const double w = (tick - e.wt_last) * c.dt;
+                e.wt_sum += w;
+                e.wt_sum2 = fma(w, w, e.wt_sum2);
+                if (e.wt_n == 0 || w > e.wt_max) e.wt_max = w;
+                e.wt_n++;
             }
-            o->wt_last = tick;
-            o->wt_ticks++;
-            o->wt_passed = 0;
+            e.wt_last = tick;
+            e.wt_ticks++;
+            e.wt_passed = 0;
         }
     }
     if (c.b_fpt) {
         if (single) {
-            if (o->fpt_to_update && s.ent_period) {
-                o->fpt_to_update = 0;
-                o->fpt_last_tick = t;
-                fpt1_arm(c, f, P, traj, w, o, s, err);
+            if (e.fpt_to_update && s.ent_period) {
+                e.fpt_to_update = 0;
+                e.fpt_last_tick = t;
+                fpt1_arm(c, f, P, traj, w, e.fpt_init_index, e.fpt_init_state, s, err);
             }
-            if (!o->fpt_to_update && s.ent_well) {
-                o->fpt_to_update = 1;
-                const double d = duration(c, iStep, o->fpt_last_tick);
-                o->fpt_sum += d;
+            if (!e.fpt_to_update && s.ent_well) {
+                e.fpt_to_update = 1;
+                const double d = duration(c, iStep, e.fpt_last_tick);
+                e.fpt_sum += d;
                 if (ev) {
                     // the barrier logs (Single.h:2398-2404)
-                    push_event(c, ev, LDPCP_LIST_FPT, o->fpt_n, d);
-                    push_event(c, ev, LDPCP_LIST_INITIAL, o->fpt_n,
-                               s.periodic == (double)o->fpt_init_index[0] ? o->fpt_init_state[0]
-                                                                           : o->fpt_init_state[1]);
-                    push_event(c, ev, LDPCP_LIST_INSTANT, o->fpt_n, s.state);
+                    push_event(c, ev, LDPCP_LIST_FPT, e.fpt_n, d);
+                    push_event(c, ev, LDPCP_LIST_INITIAL, e.fpt_n,
+                               s.periodic == (double)e.fpt_init_index[0] ? e.fpt_init_state[0]
+                                                                           : e.fpt_init_state[1]);
+                    push_event(c, ev, LDPCP_LIST_INSTANT, e.fpt_n, s.state);
                 }
-                o->fpt_n++;
+                e.fpt_n++;
             }
         }
-        if (o->fpt2_to_update && s.ent_period) {
-            o->fpt2_to_update = 0;
-            o->fpt2_last_tick = t;
+        if (e.fpt2_to_update && s.ent_period) {
+            e.fpt2_to_update = 0;
+            e.fpt2_last_tick = t;
             int idx[2];
             arm_indices(idx, s);
-            o->fpt2_idx[0] = idx[0];
-            o->fpt2_idx[1] = idx[1];
+            e.fpt2_idx[0] = idx[0];
+            e.fpt2_idx[1] = idx[1];
         }
-        if (s.ent_period && !o->fpt2_to_update && ((double)o->fpt2_idx[0] != s.periodic) &&
-            ((double)o->fpt2_idx[1] != s.periodic)) {
-            const double d = duration(c, iStep, o->fpt2_last_tick);
-            o->fpt2_sum += d;
-            if (ev) push_event(c, ev, LDPCP_LIST_FPT2, o->fpt2_n, d);
-            o->fpt2_n++;
-            o->fpt2_last_tick = t;
+        if (s.ent_period && !e.fpt2_to_update && ((double)e.fpt2_idx[0] != s.periodic) &&
+            ((double)e.fpt2_idx[1] != s.periodic)) {
+            const double d = duration(c, iStep, e.fpt2_last_tick);
+            e.fpt2_sum += d;
+            if (ev) push_event(c, ev, LDPCP_LIST_FPT2, e.fpt2_n, d);
+            e.fpt2_n++;
+            e.fpt2_last_tick = t;
             int idx[2];
             arm_indices(idx, s);
-            o->fpt2_idx[0] = idx[0];
-            o->fpt2_idx[1] = idx[1];
+            e.fpt2_idx[0] = idx[0];
+            e.fpt2_idx[1] = idx[1];
         }
     }
     if (c.b_tpt && single) {
         if (s.ent_period) {
-            o->tpt_to_update = 0;
-            o->tpt_last_tick = t;
+            e.tpt_to_update = 0;
+            e.tpt_last_tick = t;
         }
-        if (!o->tpt_to_update && s.ent_well) {
-            o->tpt_to_update = 1;
-            const double d = duration(c, iStep, o->tpt_last_tick);
-            o->tpt_sum += d;
-            if (ev) push_event(c, ev, LDPCP_LIST_TPT, o->tpt_n, d);
-            o->tpt_n++;
+        if (!e.tpt_to_update && s.ent_well) {
+            e.tpt_to_update = 1;
+            const double d = duration(c, iStep, e.tpt_last_tick);
+            e.tpt_sum += d;
+            if (ev) push_event(c, ev, LDPCP_LIST_TPT, e.tpt_n, d);
+            e.tpt_n++;
         }
         if (s.ent_period) {
-            if (o->tpt2_to_update) {
-                o->tpt2_to_update = 0;
-            } else if (s.well != (double)o->tpt2_initial_well) {
-                const double d = duration(c, iStep, o->tpt2_last_tick);
-                o->tpt2_sum += d;
-                if (ev) push_event(c, ev, LDPCP_LIST_TPT2, o->tpt2_n, d);
-                o->tpt2_n++;
+            if (e.tpt2_to_update) {
+                e.tpt2_to_update = 0;
+            } else if (s.well != (double)e.tpt2_initial_well) {
+                const double d = duration(c, iStep, e.tpt2_last_tick);
+                e.tpt2_sum += d;
+                if (ev) push_event(c, ev, LDPCP_LIST_TPT2, e.tpt2_n, d);
+                e.tpt2_n++;
             }
-            o->tpt2_last_tick = t;
-            o->tpt2_initial_well = (int)s.well;
+            e.tpt2_last_tick = t;
+            e.tpt2_initial_well = (int)s.well;
         }
     }
     if (c.b_cg && s.ent_period) {
         for (int j = 0; j < c.n_cg; ++j) {
-            if ((int)rint(s.well) % c.cg_size[j] != 0) continue;
+            if (!cg_divides(c, j, s.well)) continue;
             if (o->cg_to_update[j]) {
                 o->cg_to_update[j] = 0;
                 o->cg_last_tick[j] = t;
@@ -735,6 +798,7 @@ __device__ __forceinline__ void stats_event(const RunConst& c, const Flags& f, c
             }
         }
     }
+    store_event_state(o, e);
 }
 
 // Record (Single.h:1707-1765); frames are [field][frame][traj]
