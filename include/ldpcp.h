@@ -126,7 +126,11 @@ int ldpcp_run_integrate(ldpcp_run* h, uint64_t steps_per_launch);
  * handle's own stream and returns; ldpcp_run_wait waits for it and reports
  * per-trajectory errors.  Handles launched one after another run concurrently
  * on the GPU (the reference's MPI ranks integrate at the same time,
- * langevinDynamicsSingleBarrier.h:3429-3446).  Reading results waits first. */
+ * langevinDynamicsSingleBarrier.h:3429-3446).  A run is split into launches of
+ * up to steps_per_launch (0: 2^20) steps; ldpcp_run_launch returns once they
+ * are enqueued, waiting while the stream's hardware queue is full, so long runs
+ * of several handles are launched from one thread per handle.  Reading results
+ * waits first. */
 int ldpcp_run_launch(ldpcp_run* h, uint64_t steps_per_launch);
 int ldpcp_run_wait(ldpcp_run* h);
 
