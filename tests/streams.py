@@ -34,6 +34,8 @@ def run_rank_recorded(oracle, p, W, seed, bmin=-300, bn=600, cap=None, bank=4096
     cap = cap or (nsteps_eff + 8)
     banks = NumpyBanks(seed)
     st = oracle.ldo_rank_create(C.byref(p), W, C.cast(banks.cb, C.c_void_p), None, bank)
+    if not st:
+        raise ValueError(f"oracle rank of {W} lanes refused (1 to 16 lanes)")
     lanes = (ol.Lane * W)()
     nfr = nsteps_eff // p.save_freq + 1
     fields = 7 if p.save_all else 1

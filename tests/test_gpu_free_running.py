@@ -1,0 +1,76 @@
+"""Free-running run mode against the oracle driven by an independent generator.
+
+The replay tests pin the run mode's arithmetic bit for bit on recorded noise.
+Here each side draws its own noise: the GPU run mode its Philox4x32-10 streams,
+the oracle's rank (the reference's bank-and-lane consumption order,
+RANDOMNUMBER.cpp:157-340) numpy's PCG64 normal and uniform banks.  The two are
+different realisations of the same process, so their statistics must agree:
+
+* the pooled first-passage times (FPT1, the barrier-top arrivals from a well
+  centre, Single.h:2356-2411) pass a two-sample Kolmogorov-Smirnov test at
+  p > 0.01, and the MFPTs differ by less than 1.96 combined standard errors
+  (the GPU's MFPT lies in the oracle's 95 % confidence interval, widened by
+  the GPU estimate's own error);
+* the second-neighbour first passages (FPT2, :2413-2489) and the
+  transition-path times (TPT1, :2491-2530) pass the same KS test, and their
+  means differ by less than 3 combined standard errors (secondary statistics
+  of the same runs);
+* each lane's fraction of steps with the barrier in state 1 sits at the
+  telegraph's stationary 1/2 (EQ true): their mean is within 4 standard
+  errors of it.
+
+These are the north-star's free-running criteria (the reference's 95 % CI and a
+KS p > 0.01), with the oracle standing in for the reference program.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from langevindynamicsforpcp_amd.integrator import read_checkpoint
+from streams import run_rank_recorded
+from test_gpu_run import to_engine
+from test_oracle_modes import make
+
+pytestmark = pytest.mark.gpu
+
+LISTS = {"fpt": 0, "fpt2": 1, "tpt": 2}
+
+
+def _compare(a, b, what, z_max):
+    from scipy.stats import ks_2samp
+    assert len(a) > 2000 and len(b) > 2000, (what, len(a), len(b))
+    p = ks_2samp(a, b).pvalue
+    se = math.sqrt(a.var() / len(a) + b.var() / len(b))
+    print(f"[free running] {what}: n {len(a)} / {len(b)}, mean {a.mean():.5f} / {b.mean():.5f}, "
+          f"z {(a.mean() - b.mean()) / se:+.2f}, KS p {p:.3f}", flush=True)
+    assert p > 0.01, what
+    assert abs(a.mean() - b.mean()) < z_max * se, what
+
+
+@pytest.mark.parametrize("name", ["single_over", "single_under"])
+def test_gpu_free_running_statistics_match_oracle(oracle, name):
+    nsteps, capn, W, ranks, n_gpu = 200000, 8192, 16, 32, 4096
+    p = make(name, nsteps=nsteps, save_freq=1000)
+    print(f"[free running] {name}: oracle {ranks} x {W} lanes, GPU {n_gpu} trajectories, {nsteps} steps", flush=True)
+    ref = {k: [] for k in LISTS}
+    for r in range(ranks):
+        lanes, _, s = run_rank_recorded(oracle, p, W, seed=5000 + 13 * r, cap=8, capn=capn)
+        for k in LISTS:
+            ref[k] += [s["capture"][k][i, :min(int(getattr(lanes[i], k + "_n")), capn)] for i in range(W)]
+    ref = {k: np.concatenate(v) for k, v in ref.items()}
+
+    le = to_engine(p, n_gpu, seed=99)
+    le.barrier_window = 4096
+    le.event_capacity = capn
+    le.Integrate(0)
+    img = read_checkpoint(le.Checkpoint())
+    got = {k: np.concatenate([img["events"][i][j] for i in range(n_gpu)]) for k, j in LISTS.items()}
+    for k in LISTS:
+        _compare(got[k], ref[k], k, 1.96 if k == "fpt" else 3.0)
+
+    # telegraph state: fraction of counted steps in state 1 (EQ true: 1/2)
+    gl = le.lanes()
+    frac = np.array([x.state_count[1] / (x.state_count[0] + x.state_count[1]) for x in gl])
+    assert abs(frac.mean() - 0.5) < 4 * frac.std(ddof=1) / math.sqrt(len(frac)) + 1e-3, frac.mean()
+    le.close()
