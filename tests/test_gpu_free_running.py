@@ -19,8 +19,10 @@ different realisations of the same process, so their statistics must agree:
   telegraph's stationary 1/2 (EQ true): their mean is within 4 standard
   errors of it.
 
-These are the north-star's free-running criteria (the reference's 95 % CI and a
-KS p > 0.01), with the oracle standing in for the reference program.
+The double barrier keeps FPT2 and FPTCG only; there FPT2 takes the MFPT test
+and the first FPTCG list the secondary one.  These are the north-star's
+free-running criteria (the reference's 95 % CI and a KS p > 0.01), with the
+oracle standing in for the reference program.
 """
 import math
 
@@ -34,12 +36,9 @@ from test_oracle_modes import make
 
 pytestmark = pytest.mark.gpu
 
-LISTS = {"fpt": 0, "fpt2": 1, "tpt": 2}
-
-
 def _compare(a, b, what, z_max):
     from scipy.stats import ks_2samp
-    assert len(a) > 2000 and len(b) > 2000, (what, len(a), len(b))
+    assert len(a) > 1000 and len(b) > 1000, (what, len(a), len(b))
     p = ks_2samp(a, b).pvalue
     se = math.sqrt(a.var() / len(a) + b.var() / len(b))
     print(f"[free running] {what}: n {len(a)} / {len(b)}, mean {a.mean():.5f} / {b.mean():.5f}, "
@@ -48,16 +47,30 @@ def _compare(a, b, what, z_max):
     assert abs(a.mean() - b.mean()) < z_max * se, what
 
 
-@pytest.mark.parametrize("name", ["single_over", "single_under"])
+# lists compared per model: (name, GPU event list, oracle capture key and list index, z bound);
+# the double barrier keeps FPT2 and FPTCG only (Double.h:2974-3006, 3103-3154)
+COMPARED = {
+    0: [("fpt", 0, ("fpt", None), 1.96), ("fpt2", 1, ("fpt2", None), 3.0), ("tpt", 2, ("tpt", None), 3.0)],
+    1: [("fpt2", 1, ("fpt2", None), 1.96), ("fptcg0", 6, ("cg", 0), 3.0)],
+}
+
+
+def _lane_count(lane, key, j):
+    return int(lane.cg_n[j]) if key == "cg" else int(getattr(lane, key + "_n"))
+
+
+@pytest.mark.parametrize("name", ["single_over", "single_under", "double_over", "double_under"])
 def test_gpu_free_running_statistics_match_oracle(oracle, name):
     nsteps, capn, W, ranks, n_gpu = 200000, 8192, 16, 32, 4096
     p = make(name, nsteps=nsteps, save_freq=1000)
+    lists = COMPARED[p.model]
     print(f"[free running] {name}: oracle {ranks} x {W} lanes, GPU {n_gpu} trajectories, {nsteps} steps", flush=True)
-    ref = {k: [] for k in LISTS}
+    ref = {k: [] for k, *_ in lists}
     for r in range(ranks):
         lanes, _, s = run_rank_recorded(oracle, p, W, seed=5000 + 13 * r, cap=8, capn=capn)
-        for k in LISTS:
-            ref[k] += [s["capture"][k][i, :min(int(getattr(lanes[i], k + "_n")), capn)] for i in range(W)]
+        for k, _, (key, j), _ in lists:
+            src = s["capture"][key] if j is None else s["capture"][key][j]
+            ref[k] += [src[i, :min(_lane_count(lanes[i], key, j or 0), capn)] for i in range(W)]
     ref = {k: np.concatenate(v) for k, v in ref.items()}
 
     le = to_engine(p, n_gpu, seed=99)
@@ -65,12 +78,12 @@ def test_gpu_free_running_statistics_match_oracle(oracle, name):
     le.event_capacity = capn
     le.Integrate(0)
     img = read_checkpoint(le.Checkpoint())
-    got = {k: np.concatenate([img["events"][i][j] for i in range(n_gpu)]) for k, j in LISTS.items()}
-    for k in LISTS:
-        _compare(got[k], ref[k], k, 1.96 if k == "fpt" else 3.0)
+    for k, gl, _, z in lists:
+        got = np.concatenate([img["events"][i][gl] for i in range(n_gpu)])
+        _compare(got, ref[k], k, z)
 
     # telegraph state: fraction of counted steps in state 1 (EQ true: 1/2)
-    gl = le.lanes()
-    frac = np.array([x.state_count[1] / (x.state_count[0] + x.state_count[1]) for x in gl])
+    lanes = le.lanes()
+    frac = np.array([x.state_count[1] / (x.state_count[0] + x.state_count[1]) for x in lanes])
     assert abs(frac.mean() - 0.5) < 4 * frac.std(ddof=1) / math.sqrt(len(frac)) + 1e-3, frac.mean()
     le.close()
