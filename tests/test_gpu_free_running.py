@@ -15,6 +15,7 @@ different realisations of the same process, so their statistics must agree:
   transition-path times (TPT1, :2491-2530) pass the same KS test, and their
   means differ by less than 3 combined standard errors (secondary statistics
   of the same runs);
+* the MSD over the saved frames agrees within 3 standard errors at every lag;
 * each lane's fraction of steps with the barrier in state 1 sits at the
   telegraph's stationary 1/2 (EQ true): their mean is within 4 standard
   errors of it.
@@ -66,12 +67,15 @@ def test_gpu_free_running_statistics_match_oracle(oracle, name):
     lists = COMPARED[p.model]
     print(f"[free running] {name}: oracle {ranks} x {W} lanes, GPU {n_gpu} trajectories, {nsteps} steps", flush=True)
     ref = {k: [] for k, *_ in lists}
+    ref_x = []
     for r in range(ranks):
-        lanes, _, s = run_rank_recorded(oracle, p, W, seed=5000 + 13 * r, cap=8, capn=capn)
+        lanes, frames, s = run_rank_recorded(oracle, p, W, seed=5000 + 13 * r, cap=8, capn=capn)
         for k, _, (key, j), _ in lists:
             src = s["capture"][key] if j is None else s["capture"][key][j]
             ref[k] += [src[i, :min(_lane_count(lanes[i], key, j or 0), capn)] for i in range(W)]
+        ref_x.append(frames[0].T)
     ref = {k: np.concatenate(v) for k, v in ref.items()}
+    ref_x = np.concatenate(ref_x)        # [lane][frame]
 
     le = to_engine(p, n_gpu, seed=99)
     le.barrier_window = 4096
@@ -81,6 +85,19 @@ def test_gpu_free_running_statistics_match_oracle(oracle, name):
     for k, gl, _, z in lists:
         got = np.concatenate([img["events"][i][gl] for i in range(n_gpu)])
         _compare(got, ref[k], k, z)
+
+    # MSD per trajectory over the saved frames (MeanSquareDisplacementBruteForce,
+    # Single.h:24-45): the mean over trajectories within 3 standard errors at
+    # every lag
+    lags = np.array([1, 2, 5, 10, 20, 50, 100], np.uint64)
+    m_gpu = le.msd(lags)
+    m_ref = np.array([[np.mean((x[l:] - x[:-l]) ** 2) for l in lags.astype(int)] for x in ref_x])
+    for k, l in enumerate(lags):
+        a, b = m_gpu[:, k], m_ref[:, k]
+        se = math.sqrt(a.var() / len(a) + b.var() / len(b))
+        print(f"[free running] msd lag {int(l)}: {a.mean():.4f} / {b.mean():.4f}, z {(a.mean() - b.mean()) / se:+.2f}",
+              flush=True)
+        assert abs(a.mean() - b.mean()) < 3 * se, int(l)
 
     # telegraph state: fraction of counted steps in state 1 (EQ true: 1/2)
     lanes = le.lanes()
