@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ldpcp_gauss.h"
+
 namespace ldpcp {
 
 struct Philox4 {
@@ -58,13 +60,11 @@ __host__ __device__ __forceinline__ double unit_f64(uint32_t w)
     return ((double)w + 0.5) * 2.3283064365386963e-10;
 }
 
+// Box-Muller pair from the IEEE-operation logarithm and sine/cosine of
+// ldpcp_gauss.h: the same bits on the device and in a host build.
 __host__ __device__ __forceinline__ void box_muller_f64(uint32_t a, uint32_t b, double& g0, double& g1)
 {
-    const double u1 = unit_f64(a), u2 = unit_f64(b);
-    const double r = sqrt(-2.0 * log(u1));
-    const double th = 6.283185307179586 * u2;
-    g0 = r * cos(th);
-    g1 = r * sin(th);
+    ldpcp_box_muller(a, b, &g0, &g1);
 }
 
 // Integer form of "u < p" for u = (w + 1/2) / 2^32: w < ceil(p 2^32 - 1/2).

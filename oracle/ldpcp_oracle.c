@@ -15,6 +15,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "ldpcp_gauss.h"
+
 /* ------------------------------------------------------------------------- */
 /* Constants: IntegrateInitializeVariables (Single.h:1520-1618, overdamped     */
 /* :2812-2824; Double.h:1733-1937 with GetXc :1722-1732).                      */
@@ -900,15 +902,26 @@ static void philox_role(const lane_noise* n, uint64_t idx, uint32_t role, uint32
     ldo_philox4x32_10(ctr, n->key, out);
 }
 
-/* Box-Muller on two 32-bit words: r = sqrt(-2 ln u1), theta = 2 pi u2. */
+/* Box-Muller on two 32-bit words: r = sqrt(-2 ln u1), theta = 2 pi u2, with the
+ * logarithm and sine/cosine the GPU uses (ldpcp_gauss.h: IEEE operations only, so
+ * the bits agree).  This is the build's noise definition, not the reference's
+ * (whose MKL ICDF streams the replay mode carries); the functions are pinned
+ * against a correctly rounded reference by ldo_log / ldo_sincos_turn. */
 static void box_muller(uint32_t a, uint32_t b, double* g0, double* g1)
 {
-    double u1 = u32_to_unit(a), u2 = u32_to_unit(b);
-    double r = sqrt(-2.0 * log(u1));
-    double th = 6.283185307179586 * u2;
-    *g0 = r * cos(th);
-    *g1 = r * sin(th);
+    ldpcp_box_muller(a, b, g0, g1);
 }
+
+double ldo_log(double x) { return ldpcp_log(x); }
+
+void ldo_double_apex(const ldo_params* p, double* xc, double* yc)
+{
+    consts c;
+    make_consts(p, &c);
+    *xc = c.xc;
+    *yc = c.yc;
+}
+void ldo_sincos_turn(double u, double* s, double* c) { ldpcp_sincos_turn(u, s, c); }
 
 /* role 1: initial draws; word 0/1 -> V0 (cos) and R0 (sin), word 2 -> state */
 static double noise_init_gauss(lane_noise* n, int which, ldo_lane* o)
@@ -1032,6 +1045,27 @@ static void lane_store_entry(lane_store* t, lane_noise* n, const consts* c, cons
     *slot = k;
 }
 
+/* whether barrier `id` has an entry (has been touched) */
+static int store_has(const lane_store* t, int id)
+{
+    if (t->n == 0) return 0;
+    const int k = id - t->min_id;
+    return k >= 0 && k < t->n && t->valid[k];
+}
+
+/* Test hook: lane mode appends every barrier hand-over to this trace. */
+static ldo_handover* g_trace;
+static uint64_t g_trace_cap, g_trace_n;
+
+void ldo_trace_handovers(ldo_handover* buf, uint64_t cap)
+{
+    g_trace = buf;
+    g_trace_cap = buf ? cap : 0;
+    g_trace_n = 0;
+}
+
+uint64_t ldo_trace_count(void) { return g_trace_n; }
+
 /* the statistics' barrier lookups in lane mode: the lane's own store */
 typedef struct lane_touch_ctx {
     lane_store* st;
@@ -1116,8 +1150,24 @@ static int lane_run(const ldo_params* p, ldo_lane* o, lane_noise* n, double* fra
             lane_store_entry(&st[ap], n, c, p, ap, (int)L.periodic_prev, &kp);
             st[ap].state[kp] = (int)L.state;
             st[ap].step[kp] = iStep;
+            const int first_touch = !store_has(&st[ac], (int)L.periodic);
             lane_store_entry(&st[ac], n, c, p, ac, (int)L.periodic, &kc);
             double prev_state = (double)st[ac].state[kc];
+            ldo_handover* tr = NULL;
+            if (g_trace && g_trace_n < g_trace_cap) {
+                tr = &g_trace[g_trace_n];
+                tr->step = iStep;
+                tr->x = L.x;
+                tr->left_array = ap;
+                tr->left_id = (int)L.periodic_prev;
+                tr->left_state = (int)L.state;
+                tr->entered_array = ac;
+                tr->entered_id = (int)L.periodic;
+                tr->stored_state = (int)prev_state;
+                tr->first_touch = first_touch;
+                tr->stored_step = st[ac].step[kc];
+            }
+            if (g_trace) g_trace_n++;
             if (c->tau_zero) {
                 L.state = prev_state;
             } else {
@@ -1129,6 +1179,7 @@ static int lane_run(const ldo_params* p, ldo_lane* o, lane_noise* n, double* fra
                 double uc = noise_catch(n, o);
                 L.state = (uc < pc) ? ((prev_state == 1.0) ? 0.0 : 1.0) : prev_state;
             }
+            if (tr) tr->new_state = (int)L.state;
         }
         lane_param(&L, c, model);
         lane_energy_force(&L, c, model);

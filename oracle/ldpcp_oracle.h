@@ -129,6 +129,29 @@ int ldo_lane_run_philox(const ldo_params* p, ldo_lane* out, uint64_t seed, uint3
                         double* frames_x, uint64_t frames_cap);
 
 void ldo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Test hook: while a trace buffer is set, lane mode appends one record per
+ * barrier hand-over (Single.h:2025-2068; Double.h:2466-2510); count() is the
+ * number of hand-overs since the buffer was set, also past cap. */
+typedef struct ldo_handover {
+    uint64_t step;              /* iStep of the hand-over */
+    double x;                   /* position after the step's move */
+    int32_t left_array, left_id, left_state;       /* barrier left and the state it keeps */
+    int32_t entered_array, entered_id, stored_state; /* barrier entered and its stored state */
+    int32_t first_touch;        /* 1: the entered barrier was created by this hand-over */
+    int32_t new_state;          /* state after the catch-up flip */
+    uint64_t stored_step;       /* step the entered barrier was last left (0: never) */
+} ldo_handover;
+void ldo_trace_handovers(ldo_handover* buf, uint64_t cap);
+uint64_t ldo_trace_count(void);
+
+/* The Box-Muller logarithm and sine/cosine of 2 pi u shared with the GPU
+ * (ldpcp_gauss.h), exported so the tests can pin them against a correctly
+ * rounded reference. */
+double ldo_log(double x);
+/* The double barrier's intermediate-well bottom (GetXc, Double.h:1722-1732) as
+ * the constants hold it. */
+void ldo_double_apex(const ldo_params* p, double* xc, double* yc);
+void ldo_sincos_turn(double u, double* s, double* c);
 uint64_t ldo_flip_threshold(double p);
 
 /* Post-processing helpers restated from the runner. */

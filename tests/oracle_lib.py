@@ -104,6 +104,12 @@ def load():
                                         C.POINTER(C.c_int8), I, I, D, U64, C.POINTER(Capture), I]
     lib.ldo_lane_run_philox.argtypes = [C.POINTER(Params), C.POINTER(Lane), U64, C.c_uint32, U64, U64, D, U64]
     lib.ldo_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.ldo_trace_handovers.argtypes = [C.c_void_p, U64]
+    lib.ldo_trace_count.restype = U64
+    lib.ldo_double_apex.argtypes = [C.POINTER(Params), D, D]
+    lib.ldo_log.restype = C.c_double
+    lib.ldo_log.argtypes = [C.c_double]
+    lib.ldo_sincos_turn.argtypes = [C.c_double, D, D]
     lib.ldo_flip_threshold.restype = U64
     lib.ldo_flip_threshold.argtypes = [C.c_double]
     lib.ldo_log10_scale.restype = U64

@@ -115,7 +115,7 @@ def test_single_barrier_executable_matches_oracle(oracle, tmp_path, over):
     }
     for name, want in exp.items():
         got = np.fromfile(tmp_path / f"out_{name}", "<f8")
-        np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12, err_msg=name)
+        np.testing.assert_array_equal(got, want, err_msg=name)
     # msd.log: lags from GenerateLog10Scale, per-lane time-origin MSD summed rank-major / nrun
     lags = np.zeros(512, np.uint64)
     nl = oracle.ldo_log10_scale(0.1, 2001, 1000, 20, 1, lags.ctypes.data_as(C.POINTER(C.c_uint64)), 512)
@@ -247,7 +247,7 @@ def test_double_barrier_executable_matches_oracle(oracle, tmp_path, over):
     }
     for name, want in exp.items():
         got = np.fromfile(tmp_path / f"dbl_{name}", "<f8")
-        np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12, err_msg=name)
+        np.testing.assert_array_equal(got, want, err_msg=name)
     for name in ("firstPassageTime.dat", "transitionPathTime.dat", "transitionPathTime2.dat"):
         got = np.fromfile(tmp_path / f"dbl_{name}", "<f8")
         assert got.shape == (ranks * W,) and not got.any(), name

@@ -3,9 +3,9 @@
 Replay: the GPU consumes the role-tagged noise a reference-style rank (the
 oracle's rank mode, fed by seeded banks) drew, and must reproduce the rank's
 trajectories bit for bit: every saved frame, the end state, every statistics
-sum and count.  Philox: GPU and oracle draw the same counter-based noise; the
-two differ only by the last-ulp behaviour of log/sin/cos/exp, so positions
-agree to 1e-12 relative and event counts exactly.
+sum and count.  Philox: GPU and oracle draw the same counter-based noise and
+build its Gaussians from IEEE operations only (ldpcp_gauss.h), so every saved
+position agrees bit for bit and event counts exactly.
 """
 import ctypes as C
 
@@ -89,7 +89,7 @@ def test_gpu_philox_matches_oracle(oracle, name):
         ref = ol.Lane()
         rf = np.zeros(fr.shape[1])
         oracle.ldo_lane_run_philox(C.byref(p), C.byref(ref), 99, t, 0, 0, ol.dptr(rf), len(rf))
-        np.testing.assert_allclose(fr[t], rf, rtol=1e-12, atol=1e-12)
+        np.testing.assert_array_equal(fr[t], rf, err_msg=f"{name} trajectory {t}")
         for f in COUNTS:
             assert getattr(got[t], f) == getattr(ref, f), (name, t, f)
         assert got[t].state == ref.state

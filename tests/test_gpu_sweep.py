@@ -52,14 +52,16 @@ def test_fp64_sweep_matches_oracle(oracle, model, over):
     sw.run(steps_per_launch=5000)
     ab, tp, xf, xl = sw.paths(0)
     st = sw.stats(0)
-    same = 0
+    differ = []
     for t in range(s.n_traj):
         ref = oracle_absorb(oracle, s, t, absorb)
         ok = ref.absorbed_step == ab[t] and (ab[t] == 0 or ref.tpt_start_step == tp[t])
         if model == 1 and ab[t]:
             ok = ok and ref.xc_first_step == xf[t] and ref.xc_last_step == xl[t]
-        same += ok
-    assert same >= s.n_traj - 1, f"{s.n_traj - same} trajectories differ"
+        if not ok:
+            differ.append((t, int(ab[t]), ref.absorbed_step, int(tp[t]), ref.tpt_start_step))
+    # same Philox noise, same Box-Muller bits (IEEE operations only), same operation order
+    assert not differ, f"trajectories differ (t, gpu step, oracle step, gpu tpt0, oracle tpt0): {differ[:8]}"
     assert st.n_absorbed == int(np.count_nonzero(ab))
     assert st.n_absorbed + st.n_capped == s.n_traj
     assert sum(st.hist) == st.n_absorbed
