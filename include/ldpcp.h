@@ -155,6 +155,15 @@ int ldpcp_run_ticks(ldpcp_run* h, uint32_t begin, uint32_t n, double* out);
  * frames), in the reference's summation order. */
 int ldpcp_run_msd(ldpcp_run* h, const uint64_t* lags, uint32_t nlags, double* out);
 
+/* No counterpart in the reference: ldpcp_run_msd for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results; the lags and
+ * results stage through a fixed 8 MiB device array instead of two allocations,
+ * so n_traj * nlags must not exceed 983040 (else LDPCP_EINVAL).  The array is
+ * shared by every call on a device: calls that overlap on one device race.
+ * The handle must not have a launched run pending (wait for it first). */
+int ldpcp_run_msd_noalloc(ldpcp_run* h, const uint64_t* lags, uint32_t nlags, double* out);
+
 /* Replaces UserData::m_bInitialValue (langevinDynamicsSingleBarrier.h:1378-1380,
  * used by BarrierStatus::Initialize :80-84): with UNIFORMINITALSTATES, barriers
  * created during the run take this state per trajectory ([n_traj], 0 or 1).  It

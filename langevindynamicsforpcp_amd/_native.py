@@ -139,6 +139,7 @@ _SIGNATURES = [
     ("ldpcp_run_frames", C.c_int, [_P, C.c_int32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
     ("ldpcp_run_ticks", C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
     ("ldpcp_run_msd", C.c_int, [_P, C.POINTER(C.c_uint64), C.c_uint32, C.POINTER(C.c_double)]),
+    ("ldpcp_run_msd_noalloc", C.c_int, [_P, C.POINTER(C.c_uint64), C.c_uint32, C.POINTER(C.c_double)]),
     ("ldpcp_run_set_barrier_default", C.c_int, [_P, C.POINTER(C.c_double)]),
     ("ldpcp_run_checkpoint_size", C.c_int, [_P, C.POINTER(C.c_uint64)]),
     ("ldpcp_run_checkpoint", C.c_int, [_P, C.c_void_p, C.c_uint64]),

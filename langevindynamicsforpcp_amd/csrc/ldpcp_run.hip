@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "ldpcp_common.hpp"
@@ -1365,6 +1366,7 @@ struct ldpcp_run {
     int phase;            // 0 fresh, 1 resumed (next integrate continues), 2 resumed run done
     bool integrated;
     bool pending;         // launched, not yet waited for
+    int device;           // HIP device the handle's memory and stream live on
 };
 
 using namespace ldpcp;
@@ -1414,6 +1416,7 @@ extern "C" int ldpcp_run_create(const ldpcp_run_params* p, uint32_t n_traj, uint
     if (rc) return rc;
     ldpcp_run* h = new ldpcp_run();
     h->params = *p;
+    if (hipGetDevice(&h->device) != hipSuccess) h->device = 0;
     make_const(*p, h->c);
     h->c.n_traj = n_traj;
     h->c.traj_offset = traj_offset;
@@ -1738,6 +1741,95 @@ extern "C" int ldpcp_run_msd(ldpcp_run* h, const uint64_t* lags, uint32_t nlags,
     (void)hipFree(d_lags);
     (void)hipFree(d_out);
     return LDPCP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ldpcp_run_msd without device allocation.  ldpcp_run_msd stages the lags and
+// the per-trajectory results through two hipMalloc'd buffers; this variant
+// stages them through `scratch`, a fixed array in the library's device image
+// (each device holds its own copy for the life of the process):
+//   [0, 512 KiB)   lags (up to 65535 u64)
+//   [512 KiB, 8 MiB) results, n_traj * nlags doubles (at most 983040)
+// Every call on a device shares that device's copy, so calls that overlap on
+// one device (from several host threads) race.  Each call writes every result
+// it reads back, so nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr uint64_t kScratchBytes = 8ull << 20;
+constexpr uint64_t kScratchLagBytes = 65536ull * sizeof(uint64_t);
+constexpr uint64_t kScratchResults = (kScratchBytes - kScratchLagBytes) / sizeof(double);
+constexpr int kScratchDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on device `dev` (which must be current), looked up
+// once per device: the array's host-side name is not a device address.
+static int scratch_address(int dev, void** out)
+{
+    static std::mutex mu;
+    static void* addr[kScratchDevices] = {};
+    if (dev < 0 || dev >= kScratchDevices) {
+        set_error("ldpcp_run_msd_noalloc: device %d beyond the %d the library tracks", dev, kScratchDevices);
+        return LDPCP_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("ldpcp_run_msd_noalloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+            return LDPCP_EDEVICE;
+        }
+        addr[dev] = p;
+    }
+    *out = addr[dev];
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_run_msd_noalloc(ldpcp_run* h, const uint64_t* lags, uint32_t nlags, double* out)
+{
+    if (!h || !lags || !out || nlags == 0 || nlags > 65535) {
+        set_error("ldpcp_run_msd_noalloc: bad arguments");
+        return LDPCP_EINVAL;
+    }
+    if (h->pending) {
+        set_error("ldpcp_run_msd_noalloc: the handle has a launched run; wait for it first");
+        return LDPCP_EINVAL;
+    }
+    const uint64_t nt = h->c.n_traj;
+    if (nt * nlags > kScratchResults) {
+        set_error("ldpcp_run_msd_noalloc: %llu trajectories x %u lags exceed the %llu results the fixed "
+                  "scratch holds; use ldpcp_run_msd",
+                  (unsigned long long)nt, nlags, (unsigned long long)kScratchResults);
+        return LDPCP_EINVAL;
+    }
+    int cur = 0;
+    LDPCP_HIP_TRY(hipGetDevice(&cur));
+    if (cur != h->device) LDPCP_HIP_TRY(hipSetDevice(h->device));
+    void* base = nullptr;
+    int rc = scratch_address(h->device, &base);
+    hipError_t e = hipSuccess;
+    if (rc == LDPCP_OK) {
+        uint64_t* d_lags = (uint64_t*)base;
+        double* d_out = (double*)((char*)base + kScratchLagBytes);
+        e = hipMemcpyAsync(d_lags, lags, nlags * sizeof(uint64_t), hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) {
+            const dim3 block(256), grid((unsigned)((nt + 255) / 256), nlags);
+            hipLaunchKernelGGL(msd_kernel, grid, block, 0, h->stream, h->P.frames, h->n_frames, (uint32_t)nt,
+                               (const uint64_t*)d_lags, nlags, d_out);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out, d_out, nt * nlags * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) {
+            set_error("ldpcp_run_msd_noalloc: %s", hipGetErrorString(e));
+            rc = LDPCP_EDEVICE;
+        }
+    }
+    if (cur != h->device) (void)hipSetDevice(cur);
+    return rc;
 }
 
 // ---------------------------------------------------------------------------
