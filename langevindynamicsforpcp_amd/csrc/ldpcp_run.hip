@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "ldpcp_common.hpp"
@@ -206,11 +207,25 @@ struct Lane {
 // on the critical path of every step; each value here passes through an empty
 // asm statement, so it stays an opaque register value loaded once per launch.
 struct StateParams {
-    double H[2], S[2];                          // single barrier [state]
+    double H[2], S[2], Sn[2];                   // single barrier [state]; Sn = 0 - S
     double dh[2][2], ds[2][2][2], di[2][2][2];  // double barrier [state][barrier][side]
-    double p_flip[2];                           // per-step flip probability [state] (EQUALPROB false)
+    double dfn[2][2][2];                        // double barrier force 0 - ds
+    double p_flip[2];                           // per-step flip probability leaving state [state]
     uint64_t t_flip[2];                         // its flip_threshold
 };
+
+// Wave-uniform copy of a 64-bit value, so that what derives from it stays in
+// scalar registers.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The lanes where a != b: the compare's lane mask itself (one v_cmp; a ballot of
+// a bool would first spell the bool out in a vector register).
+__device__ __forceinline__ uint64_t lanes_ne(double a, double b) { return __builtin_amdgcn_fcmp(a, b, 14); }
 
 __device__ __forceinline__ double in_register(double v)
 {
@@ -227,15 +242,17 @@ __device__ __forceinline__ uint64_t in_register(uint64_t v)
 __device__ __forceinline__ StateParams state_params(const RunConst& c, const Flags& f)
 {
     StateParams sp;
-    sp.p_flip[0] = in_register(c.p_asym0);
-    sp.p_flip[1] = in_register(c.p_asym1);
-    sp.t_flip[0] = in_register(c.t_asym0);
-    sp.t_flip[1] = in_register(c.t_asym1);
+    // equally probable states share one flip probability (Single.h:2010-2015, 2078-2086)
+    sp.p_flip[0] = in_register(c.equal_prob ? c.p_flip : c.p_asym0);
+    sp.p_flip[1] = in_register(c.equal_prob ? c.p_flip : c.p_asym1);
+    sp.t_flip[0] = in_register(c.equal_prob ? c.t_flip : c.t_asym0);
+    sp.t_flip[1] = in_register(c.equal_prob ? c.t_flip : c.t_asym1);
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
         if (f.model == 0) {
             sp.H[st] = in_register(c.H[st]);
             sp.S[st] = in_register(c.S[st]);
+            sp.Sn[st] = in_register(0.0 - c.S[st]);
             continue;
         }
 #pragma unroll
@@ -245,6 +262,7 @@ __device__ __forceinline__ StateParams state_params(const RunConst& c, const Fla
             for (int side = 0; side < 2; ++side) {
                 sp.ds[st][b][side] = in_register(c.slope[b][st][side]);
                 sp.di[st][b][side] = in_register(c.icept[b][st][side]);
+                sp.dfn[st][b][side] = in_register(0.0 - c.slope[b][st][side]);
             }
         }
     }
@@ -328,6 +346,43 @@ __device__ __forceinline__ void lane_energy_force(Lane& s, const RunConst& c, co
         }
         if (force) s.F = F;
     }
+}
+
+// Single-barrier force for state st, from the slopes as separate values (selecting
+// between elements of one array would let the compiler turn the selection into an
+// indexed load from a stack copy of the array).  -w < xp < 0 or 0 < xp < w
+// (strict, Single.h:1292-1324) is |xp| < w with xp != 0.
+__device__ __forceinline__ double force_single(double x, const RunConst& c, double S0, double S1, double Sn0,
+                                               double Sn1, bool st)
+{
+    const double r = x * c.Linv;
+    const double xp = x - rint(r) * c.L;
+    const bool band = (fabs(xp) < c.w) && (xp != 0.0);
+    const double Sp = st ? S1 : S0;
+    const double Sm = st ? Sn1 : Sn0;
+    return band ? ((xp < 0.0) ? Sm : Sp) : 0.0;
+}
+
+// The force of EnergyAndForce for barrier state st, selected from that state's
+// slopes directly: the same values lane_param + lane_energy_force give (the
+// single barrier's slopeNeg and the double barrier's F are 0 - slope, computed
+// once per launch in StateParams), without the heights and intercepts the step
+// does not use.
+__device__ __forceinline__ double lane_force(const Lane& s, const RunConst& c, const StateParams& sp, const Flags& f,
+                                             bool st)
+{
+    if (f.model == 0) return force_single(s.x, c, sp.S[0], sp.S[1], sp.Sn[0], sp.Sn[1], st);
+    const double xp = s.xp;
+    const bool n0 = (xp < c.xb[0]) && (xp > c.lo[0]);
+    const bool p0 = (xp < c.xc) && (xp > c.xb[0]);
+    const bool n1 = (xp < c.xb[1]) && (xp > c.xc);
+    const bool p1 = (xp < c.hi[1]) && (xp > c.xb[1]);
+    double F = 0.0;  // 0 - 0 outside the barriers
+    F = n0 ? (st ? sp.dfn[1][0][0] : sp.dfn[0][0][0]) : F;
+    F = p0 ? (st ? sp.dfn[1][0][1] : sp.dfn[0][0][1]) : F;
+    F = n1 ? (st ? sp.dfn[1][1][0] : sp.dfn[0][1][0]) : F;
+    F = p1 ? (st ? sp.dfn[1][1][1] : sp.dfn[0][1][1]) : F;
+    return F;
 }
 
 // InitializePeriodicIndices / UpdatePeriodicIndices (Single.h:1838-1884; Double.h:2150-2235)
@@ -1025,6 +1080,7 @@ __device__ __noinline__ int stats_event_call(const RunArgs* args, uint32_t t, Wi
 // computed in parallel over all trajectory-steps beforehand).
 constexpr int kNoisePhilox = 0, kNoiseReplay = 1, kNoiseBuffered = 2;
 constexpr int kStage = 32;  // steps per LDS stage of the buffered noise
+constexpr int kStageRows = kStage + 2;  // rows of a stage array: two rows of read-ahead padding
 // s_waitcnt immediate (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4,
 // lgkmcnt 11:8) that waits for vmcnt = 0 and lgkmcnt = 0 and leaves expcnt free
 constexpr int kWaitVmLgkm0 = 0x0070;
@@ -1073,32 +1129,53 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         w.dirty = win_cache_dirty;
     }
     uint64_t n_catch = o->n_catch;
-    // frames every save_freq steps: a countdown instead of a 64-bit modulo per step
-    uint64_t to_save = c.save_freq - s0 % c.save_freq;
+    // frames every save_freq steps: the step of the next frame (wave-uniform, so the
+    // test is a scalar compare)
+    uint64_t next_save = (s0 / c.save_freq + 1) * c.save_freq;
     uint64_t frame = s0 / c.save_freq + 1;
+    // The barrier state as an integer, and the steps of this launch counted in
+    // state 1 (StateCount, Single.h:2154-2165; a launch has at most 2^20 steps);
+    // s.state, s.cnt0 and s.cnt1 are brought up to date where they are stored.
+    // The state and the `dead` flag are plain integers in vector registers: as
+    // lane masks, every join after a rare update would cost mask merges on the
+    // common path.
+    int st = (s.state == 1.0) ? 1 : 0;
+    uint32_t c1 = 0;
 
     // The step loop runs all of the launch's steps for every trajectory (uniform
     // control flow).  A trajectory whose launch ends with an error stores its
-    // record and state at that point and is no longer live: its later steps have
-    // no effect outside registers.
-    bool live = true;
-    auto finish = [&](uint64_t last) {
+    // record and state at that point and is dead: its later steps have no effect
+    // outside registers.
+    int dead = 0;
+    // stores the state after step `last`, with `counted` steps of the launch in the state counts
+    auto finish = [&](uint64_t last, uint64_t counted) __attribute__((always_inline)) {
+        s.state = st ? 1.0 : 0.0;
+        s.cnt1 += c1;
+        s.cnt0 += counted - c1;
         o->n_catch = n_catch;
         o->n_gauss += last - s0;
         if (!f.tau0) o->n_flip += last - s0;
         o->steps_done = last;
         o->error = err;
+        lane_param(s, sp, f, false);
         lane_energy_force(s, c, f, false, true);
         store_lane(s, P, t);
     };
-    auto fail = [&](uint64_t iStep) {
-        finish(iStep - 1);
-        live = false;
+    auto fail = [&](uint64_t iStep, uint64_t counted) __attribute__((always_inline)) {
+        finish(iStep - 1, counted);
+        dead = 1;
     };
 
     // One step with its noise: the Gaussian g and the flip uniform uw, either as
     // its 32-bit Philox word (tested against flip_threshold) or, replayed, as a double.
-    auto step = [&](uint64_t iStep, double g, auto uw) {
+    // The common step is straight-line code; the hand-over and the statistics
+    // event sit behind one wave-uniform test (the lanes whose indices changed),
+    // taken on the few steps where some trajectory of the wavefront needs them.
+    // `frames` (a compile-time flag) adds the frame test; a caller passes it only
+    // for runs of steps that hold a frame step.
+    // `istep` returns the step number: called only where the number is used (the rare
+    // branch and the frame test), so the common step does not compute it.
+    auto step = [&](auto istep, double g, auto uw, auto eq, auto frames) __attribute__((always_inline)) {
         if (f.over) {
             const double rf = c.var * g;
             s.x = fma(s.F, c.dtgm, s.x);
@@ -1113,121 +1190,172 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         }
         lane_indices(s, c, f, false);
         // per-step flip of the barrier under the particle (Single.h:2010-2015, 2078-2086);
-        // the state is 0.0 or 1.0, so a flip is 1 - state
+        // equally probable states share one threshold, below 2^31 (p <= 1/2)
         if (!f.tau0) {
-            const bool one = (s.state == 1.0);
             bool flip;
-            if constexpr (sizeof(uw) == sizeof(uint32_t))
-                flip = (uint64_t)uw < (c.equal_prob ? c.t_flip : (one ? sp.t_flip[1] : sp.t_flip[0]));
-            else
-                flip = uw < (c.equal_prob ? c.p_flip : (one ? sp.p_flip[1] : sp.p_flip[0]));
-            s.state = flip ? 1.0 - s.state : s.state;
+            if constexpr (sizeof(uw) == sizeof(uint32_t)) {
+                if constexpr (decltype(eq)::value) flip = uw < (uint32_t)c.t_flip;
+                else flip = (uint64_t)uw < (st ? sp.t_flip[1] : sp.t_flip[0]);
+            } else {
+                if constexpr (decltype(eq)::value) flip = uw < c.p_flip;
+                else flip = uw < (st ? sp.p_flip[1] : sp.p_flip[0]);
+            }
+            st = flip ? st ^ 1 : st;
         }
-        const bool entering = f.model == 0 ? s.ent_period : s.ent_local;
-        if (entering && live) {
-            const HandOver r = hand_over_call<MODEL, OVER, TAU0, REPLAY>(
-                args, t, w, iStep, s.periodic_prev, s.periodic, s.local_prev, s.local, s.state, n_catch);
-            s.state = r.state;
-            n_catch = r.n_catch;
-            err = r.err;
-            if (err) fail(iStep);
+        double F = lane_force(s, c, sp, f, st != 0);
+        c1 += (uint32_t)st;
+        const uint64_t m_period = lanes_ne(s.periodic, s.periodic_prev);
+        const uint64_t m_rare = m_period | lanes_ne(s.well, s.well_prev) |
+                                (f.model == 0 ? 0ull : lanes_ne(s.local, s.local_prev));
+        if (__builtin_expect(m_rare != 0, 0)) {
+            const uint64_t iStep = istep();
+            const bool live = (dead == 0);
+            const bool entering = f.model == 0 ? s.ent_period : s.ent_local;
+            if (entering && live) {
+                const HandOver r = hand_over_call<MODEL, OVER, TAU0, REPLAY>(
+                    args, t, w, iStep, s.periodic_prev, s.periodic, s.local_prev, s.local, st ? 1.0 : 0.0, n_catch);
+                n_catch = r.n_catch;
+                err = r.err;
+                if (err) {
+                    c1 -= (uint32_t)st;  // this step is not counted
+                    fail(iStep, iStep - 1 - s0);
+                } else {
+                    const int nst = (r.state == 1.0) ? 1 : 0;
+                    c1 += (uint32_t)nst - (uint32_t)st;
+                    st = nst;
+                    F = lane_force(s, c, sp, f, st != 0);
+                }
+            }
+            if ((s.ent_period || s.ent_well) && dead == 0) {
+                err = stats_event_call<MODEL, OVER, TAU0, REPLAY>(args, t, w, o, iStep, s.well, s.periodic,
+                                                                  st ? 1.0 : 0.0, s.ent_well, s.ent_period);
+                if (err) fail(iStep, iStep - s0);
+            }
         }
-        lane_param(s, sp, f);
-        lane_energy_force(s, c, f, true, false);
+        s.F = F;
         if (!f.over) s.v = s.v + (c.minv * s.F) * c.halfdt;
-        if ((s.ent_period || s.ent_well) && live) {
-            err = stats_event_call<MODEL, OVER, TAU0, REPLAY>(args, t, w, o, iStep, s.well, s.periodic, s.state,
-                                                              s.ent_well, s.ent_period);
-            if (err) fail(iStep);
-        }
-        if (--to_save == 0) {
-            if (live) record(c, f, P, t, s, frame);
-            frame++;
-            to_save = c.save_freq;
+        if constexpr (decltype(frames)::value) {
+            if (istep() == next_save) {
+                if (dead == 0) {
+                    s.state = st ? 1.0 : 0.0;
+                    lane_param(s, sp, f, false);
+                    record(c, f, P, t, s, frame);
+                }
+                frame++;
+                next_save += c.save_freq;
+            }
         }
     };
 
-    const uint64_t len = s1 - s0;
-    if constexpr (SRC == kNoiseBuffered) {
-        // Stage q of the launch (steps s0 + 1 + 32 q ...) sits in stage_a (q even) or
-        // stage_b (q odd), loaded by direct global->LDS loads while the stage before
-        // it runs.  The even and odd stages are separate code over separate arrays,
-        // so the compiler sees that a stage's LDS reads never touch the array the
-        // next stage's loads are landing in and waits for those loads only at the
-        // stage boundary.  The launch is one wavefront per workgroup (64 threads),
-        // so a boundary's __syncthreads waits for this wavefront alone.
-        __shared__ uint4 stage_a[kStage][64], stage_b[kStage][64];
-        const uint32_t lane = threadIdx.x;
-        auto load_stage = [&](uint4 (&buf)[kStage][64], uint64_t q) {
-            const uint64_t first = q * kStage;
-            const uint64_t m = first >= len ? 0 : (len - first < kStage ? len - first : kStage);
-            for (uint64_t j = 0; j < m; ++j)
-                __builtin_amdgcn_global_load_lds((const void*)(noise + (first + j) * c.n_traj + t),
-                                                 (__attribute__((address_space(3))) void*)&buf[j][0], 16, 0, 0);
-        };
-        // steps of stage q from buf; each step reads the next step's record, so
-        // the LDS latency overlaps a step
-        auto run_stage = [&](uint4 (&buf)[kStage][64], uint64_t q) {
-            const uint64_t first = q * kStage;
-            const uint32_t m = (uint32_t)(len - first < kStage ? len - first : kStage);
-            uint4 rec = buf[0][lane];
-            for (uint32_t j = 0; j < m; ++j) {
-                uint4 next = rec;
-                if (j + 1 < m) next = buf[j + 1][lane];
+    // the buffered noise stages (declared once: `run` below has two instantiations)
+    __shared__ uint4 stage_a[kStageRows][64], stage_b[kStageRows][64];
+    auto run = [&](auto eq) __attribute__((always_inline)) {
+        const uint64_t len = s1 - s0;
+        if constexpr (SRC == kNoiseBuffered) {
+            // Stage q of the launch (steps s0 + 1 + 32 q ...) sits in stage_a (q even) or
+            // stage_b (q odd), loaded by direct global->LDS loads while the stage before
+            // it runs.  The even and odd stages are separate code over separate arrays,
+            // so the compiler sees that a stage's LDS reads never touch the array the
+            // next stage's loads are landing in and waits for those loads only at the
+            // stage boundary.  The launch is one wavefront per workgroup (64 threads),
+            // so a boundary's __syncthreads waits for this wavefront alone.
+            const uint32_t lane = threadIdx.x;
+            auto load_stage = [&](uint4 (&buf)[kStageRows][64], uint64_t q) {
+                const uint64_t first = q * kStage;
+                const uint64_t m = first >= len ? 0 : (len - first < kStage ? len - first : kStage);
+                for (uint64_t j = 0; j < m; ++j)
+                    __builtin_amdgcn_global_load_lds((const void*)(noise + (first + j) * c.n_traj + t),
+                                                     (__attribute__((address_space(3))) void*)&buf[j][0], 16, 0, 0);
+            };
+            // steps of stage q from buf, two per iteration with two record registers:
+            // each step's record is read during the step before (the LDS latency overlaps
+            // a step), and the values a step carries to the next alternate registers
+            // instead of being copied.  A stage array has two rows beyond its kStage
+            // steps, so the reads one and two rows ahead stay inside it; those rows
+            // are never used.
+            auto stage_steps = [&](uint4 (&buf)[kStageRows][64], uint64_t base, uint32_t m, auto frames)
+                                   __attribute__((always_inline)) {
+                const uint4* rec = &buf[0][lane];
+                uint4 ra = rec[0];
+                const uint32_t pairs = m >> 1;
+                for (uint32_t k = 0; k < pairs; ++k) {
+                    // rows 2k + 1 and 2k + 2 (the last read of an even stage is padding row m)
+                    const uint4 rb = rec[64];
+                    double g;
+                    uint32_t fw;
+                    buffered_noise(ra, g, fw);
+                    step([&] { return base + 2 * k + 1; }, g, fw, eq, frames);
+                    ra = rec[128];
+                    buffered_noise(rb, g, fw);
+                    step([&] { return base + 2 * k + 2; }, g, fw, eq, frames);
+                    rec += 128;
+                }
+                if (m & 1) {
+                    double g;
+                    uint32_t fw;
+                    buffered_noise(ra, g, fw);
+                    step([&] { return base + m; }, g, fw, eq, frames);
+                }
+            };
+            // a stage holding a frame step runs with the frame test, the others without
+            auto run_stage = [&](uint4 (&buf)[kStageRows][64], uint64_t q) __attribute__((always_inline)) {
+                const uint64_t first = q * kStage;
+                const uint32_t m = (uint32_t)(len - first < kStage ? len - first : kStage);
+                const uint64_t base = uniform_u64(s0 + first);
+                if (next_save <= base + m) stage_steps(buf, base, m, std::true_type{});
+                else stage_steps(buf, base, m, std::false_type{});
+            };
+            // stage q has landed (and every earlier LDS read is done): load stage q + 1
+            // into the other array, whose stage q - 1 has been read
+            auto boundary = [&]() {
+                __builtin_amdgcn_s_waitcnt(kWaitVmLgkm0);
+                __syncthreads();
+            };
+            load_stage(stage_a, 0);
+            for (uint64_t q = 0; q * kStage < len; q += 2) {
+                boundary();
+                load_stage(stage_b, q + 1);
+                run_stage(stage_a, q);
+                if ((q + 1) * kStage >= len) break;
+                boundary();
+                load_stage(stage_a, q + 2);
+                run_stage(stage_b, q + 1);
+            }
+        } else if constexpr (REPLAY) {
+            // the noise of the next kAhead steps is in flight while a step runs
+            // (indices clamp to s1, which the launch keeps above s0)
+            constexpr int kAhead = 4;
+            double gq[kAhead], uq[kAhead];
+    #pragma unroll
+            for (int d = 0; d < kAhead; ++d)
+                replay_noise(c, f, P, t, (s0 + 1 + d < s1) ? s0 + 1 + d : s1, gq[d], uq[d]);
+            for (uint64_t iStep = s0 + 1; iStep <= s1; ++iStep) {
+                const double g = gq[0], u = uq[0];
+    #pragma unroll
+                for (int d = 0; d + 1 < kAhead; ++d) {
+                    gq[d] = gq[d + 1];
+                    uq[d] = uq[d + 1];
+                }
+                replay_noise(c, f, P, t, (iStep + kAhead < s1) ? iStep + kAhead : s1, gq[kAhead - 1], uq[kAhead - 1]);
+                step([&] { return iStep; }, g, u, eq, std::true_type{});
+            }
+        } else {
+            StepNoise sn;
+            sn.q = 0;
+            sn.g1 = 0.0;
+            sn.w3 = 0;
+            for (uint64_t iStep = s0 + 1; iStep <= s1; ++iStep) {
                 double g;
                 uint32_t fw;
-                buffered_noise(rec, g, fw);
-                step(s0 + first + j + 1, g, fw);
-                rec = next;
+                step_noise(c, t, iStep, sn, g, fw);
+                step([&] { return iStep; }, g, fw, eq, std::true_type{});
             }
-        };
-        // stage q has landed (and every earlier LDS read is done): load stage q + 1
-        // into the other array, whose stage q - 1 has been read
-        auto boundary = [&]() {
-            __builtin_amdgcn_s_waitcnt(kWaitVmLgkm0);
-            __syncthreads();
-        };
-        load_stage(stage_a, 0);
-        for (uint64_t q = 0; q * kStage < len; q += 2) {
-            boundary();
-            load_stage(stage_b, q + 1);
-            run_stage(stage_a, q);
-            if ((q + 1) * kStage >= len) break;
-            boundary();
-            load_stage(stage_a, q + 2);
-            run_stage(stage_b, q + 1);
         }
-    } else if constexpr (REPLAY) {
-        // the noise of the next kAhead steps is in flight while a step runs
-        // (indices clamp to s1, which the launch keeps above s0)
-        constexpr int kAhead = 4;
-        double gq[kAhead], uq[kAhead];
-#pragma unroll
-        for (int d = 0; d < kAhead; ++d)
-            replay_noise(c, f, P, t, (s0 + 1 + d < s1) ? s0 + 1 + d : s1, gq[d], uq[d]);
-        for (uint64_t iStep = s0 + 1; iStep <= s1; ++iStep) {
-            const double g = gq[0], u = uq[0];
-#pragma unroll
-            for (int d = 0; d + 1 < kAhead; ++d) {
-                gq[d] = gq[d + 1];
-                uq[d] = uq[d + 1];
-            }
-            replay_noise(c, f, P, t, (iStep + kAhead < s1) ? iStep + kAhead : s1, gq[kAhead - 1], uq[kAhead - 1]);
-            step(iStep, g, u);
-        }
-    } else {
-        StepNoise sn;
-        sn.q = 0;
-        sn.g1 = 0.0;
-        sn.w3 = 0;
-        for (uint64_t iStep = s0 + 1; iStep <= s1; ++iStep) {
-            double g;
-            uint32_t fw;
-            step_noise(c, t, iStep, sn, g, fw);
-            step(iStep, g, fw);
-        }
-    }
-    if (live) finish(s1);
+    };
+    // one copy of the loops per flip rule (uniform: a kernel argument)
+    if (c.equal_prob) run(std::true_type{});
+    else run(std::false_type{});
+    if (dead == 0) finish(s1, s1 - s0);
     if constexpr (kLocal) {
         *og = *o;
         win_flush(w);
