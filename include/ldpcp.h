@@ -197,8 +197,11 @@ typedef struct ldpcp_checkpoint_header {
     int32_t model, overdamped;
     int32_t barrier_window, n_fields;
     uint64_t seed;
-    uint64_t steps_done;                /* absolute step of the state */
-    uint64_t n_frames;                  /* steps_done / save_freq + 1 */
+    uint64_t steps_done;                /* step clock the next run continues from: the runs' NUMBEROFSTEPS
+                                           total, which exceeds the last integrated step by the trailing
+                                           NUMBEROFSTEPS mod FREQSAVE steps a run drops when FREQSAVE >= 10
+                                           (langevinDynamicsSingleBarrier.h:1397-1420) */
+    uint64_t n_frames;                  /* frames recorded: 1 + NUMBEROFSTEPS / FREQSAVE per run, summed */
     uint64_t save_freq;
     uint64_t tick_capacity;
     double timestep;
@@ -229,9 +232,9 @@ int ldpcp_run_checkpoint_size(ldpcp_run* h, uint64_t* bytes);
 int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes);
 /* Loads an image into a handle created with the same parameters, trajectory
  * count and offset (NSTEPS may differ: it is the number of further steps).  The
- * next ldpcp_run_integrate continues from the image's step, appending frames to
- * the image's; ldpcp_run_n_frames then counts both.  Such a handle integrates
- * once. */
+ * next ldpcp_run_integrate continues the image's step clock (steps_done + 1 on),
+ * recording frames every FREQSAVE steps of its own after the image's;
+ * ldpcp_run_n_frames then counts both.  Such a handle integrates once. */
 int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes);
 
 /* Replaces the barrier table the reference's restart file lists

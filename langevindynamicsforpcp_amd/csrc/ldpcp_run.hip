@@ -11,16 +11,19 @@
 #include <hip/hip_runtime.h>
 #include <limits.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <mutex>
 #include <type_traits>
 #include <vector>
 
 #include "ldpcp_common.hpp"
 #include "ldpcp_device.h"
+#include "ldpcp_schedule.h"
 
 namespace ldpcp {
 
@@ -44,6 +47,10 @@ struct RunConst {
     double slope[2][2][2], icept[2][2][2];
     double vsigma, x0, v0, initial_state;
     uint64_t save_freq, nsteps_eff, frames_cap;
+    // this run's step clock (ldpcp_schedule.h): steps run_begin+1 .. run_end are
+    // integrated, frames fall at run_begin + k save_freq with indices from frame0,
+    // and the clock ends at clock_end (the dropped trailing steps skipped)
+    uint64_t run_begin, run_end, clock_end, frame0;
     uint32_t key0, key1, n_traj, traj_offset;
     uint64_t n_gauss, n_flip, catch_cap, init_gauss, init_flip, tick_cap;
     uint64_t ev_cap;   // entries kept per event list per trajectory
@@ -65,13 +72,6 @@ struct RunPtrs {
     double* ticks;       // barrier ticks [traj][tick_cap]
     double* bdefault;    // state of barriers created with UNIFORMINITALSTATES [traj]
     double* events;      // event lists [traj][list][ev_cap]
-};
-
-// The kernel arguments as one block in device memory (a handle's copy, written at
-// each launch), for the step kernel's out-of-line branches.
-struct RunArgs {
-    RunConst c;
-    RunPtrs P;
 };
 
 // The run's branch flags, separate from RunConst so that a kernel can make them
@@ -177,8 +177,12 @@ void make_const(const ldpcp_run_params& p, RunConst& c)
     const double m = fmod(fmod(p.x0, p.periodic_length) + p.periodic_length, p.periodic_length);
     c.centre = (m == p.periodic_length * 0.5);
     c.save_freq = p.save_freq;
-    c.nsteps_eff = p.save_freq >= 10 ? (p.nsteps / p.save_freq) * p.save_freq : p.nsteps;
+    c.nsteps_eff = ldpcp_steps_integrated(p.nsteps, p.save_freq);
     c.frames_cap = c.nsteps_eff / p.save_freq + 1;
+    c.run_begin = 0;
+    c.run_end = c.nsteps_eff;
+    c.clock_end = p.nsteps;
+    c.frame0 = 1;
     c.key0 = (uint32_t)p.seed;
     c.key1 = (uint32_t)(p.seed >> 32);
     c.init_gauss = (uint64_t)(p.randomize_v ? 1 : 0) + (uint64_t)(p.overdamped ? 1 : 0);
@@ -973,11 +977,11 @@ __global__ void __launch_bounds__(256) run_init_kernel(RunConst c, RunPtrs P)
     store_lane(s, P, t);
 }
 
-// The step's rare branches, a hand-over and a statistics event, as functions of
-// their own: called on the few steps that need them (a hand-over or an event on
-// any trajectory of the wavefront: ~3 % of the steps of the quick test's
-// overdamped 16 trajectories), they keep their registers and branches out of the
-// step loop.  They read the run's constants and pointers from `args`.
+// The step's rare branches, a hand-over and a statistics event, taken on the few
+// steps where some trajectory of the wavefront has one (0.7 % of the steps of the
+// quick test's 4-trajectory ranks).  They are inline: the step loop reaches them
+// through one wave-uniform branch, and inline they use the kernel's constants
+// in registers and address the LDS tables directly.
 
 struct HandOver {
     double state;
@@ -985,29 +989,13 @@ struct HandOver {
     int err;
 };
 
-// `args` is the same for every lane: as a scalar address into constant memory its
-// fields are read with scalar loads
-using ConstArgs = const __attribute__((address_space(4))) RunArgs*;
-__device__ __forceinline__ ConstArgs uniform_args(const RunArgs* p)
-{
-    const uint64_t v = (uint64_t)p;
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return (ConstArgs)(((uint64_t)hi << 32) | lo);
-}
-
 // Hand-over to the barrier just entered (Single.h:2025-2068; Double.h:2466-2510):
 // the barrier left keeps its state and the step, the barrier entered takes its
 // stored state and catches up on the steps since it was left.
-template <int MODEL, int OVER, int TAU0, int REPLAY>
-__device__ __noinline__ HandOver hand_over_call(const RunArgs* args, uint32_t t, Window w, uint64_t iStep,
-                                                double periodic_prev, double periodic, double local_prev,
-                                                double local, double state, uint64_t n_catch)
+__device__ __forceinline__ HandOver hand_over(const RunConst& c, const RunPtrs& P, const Flags& f, uint32_t t,
+                                             const Window& w, uint64_t iStep, double periodic_prev, double periodic,
+                                             double local_prev, double local, double state, uint64_t n_catch)
 {
-    constexpr Flags f{MODEL, OVER, TAU0, REPLAY};
-    const ConstArgs a = uniform_args(args);
-    const RunConst& c = (const RunConst&)a->c;
-    const RunPtrs& P = (const RunPtrs&)a->P;
     HandOver r{state, n_catch, 0};
     const int ap = f.model == 0 ? 0 : (int)local_prev;
     const int ac = f.model == 0 ? 0 : (int)local;
@@ -1044,15 +1032,10 @@ __device__ __noinline__ HandOver hand_over_call(const RunArgs* args, uint32_t t,
 }
 
 // stats_event of a step with ent_well or ent_period; returns its error code
-template <int MODEL, int OVER, int TAU0, int REPLAY>
-__device__ __noinline__ int stats_event_call(const RunArgs* args, uint32_t t, Window w, ldpcp_lane* o, uint64_t iStep,
-                                             double well, double periodic, double state, bool ent_well,
-                                             bool ent_period)
+__device__ __forceinline__ int stats_event_at(const RunConst& c, const RunPtrs& P, const Flags& f, uint32_t t,
+                                              const Window& w, ldpcp_lane* o, uint64_t iStep, double well,
+                                              double periodic, double state, bool ent_well, bool ent_period)
 {
-    constexpr Flags f{MODEL, OVER, TAU0, REPLAY};
-    const ConstArgs a = uniform_args(args);
-    const RunConst& c = (const RunConst&)a->c;
-    const RunPtrs& P = (const RunPtrs&)a->P;
     Lane s;  // the fields the statistics read
     s.well = well;
     s.periodic = periodic;
@@ -1095,7 +1078,7 @@ __device__ __forceinline__ void buffered_noise(const uint4& r, double& g, uint32
 
 template <int MODEL, int OVER, int TAU0, int SRC>
 __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, uint64_t s0, uint64_t s1,
-                                                       const uint4* __restrict__ noise, const RunArgs* args)
+                                                       const uint4* __restrict__ noise)
 {
     constexpr int REPLAY = (SRC == kNoiseReplay);
     constexpr Flags f{MODEL, OVER, TAU0, REPLAY};
@@ -1131,8 +1114,8 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     uint64_t n_catch = o->n_catch;
     // frames every save_freq steps: the step of the next frame (wave-uniform, so the
     // test is a scalar compare)
-    uint64_t next_save = (s0 / c.save_freq + 1) * c.save_freq;
-    uint64_t frame = s0 / c.save_freq + 1;
+    uint64_t next_save, frame;
+    ldpcp_next_frame(c.run_begin, c.frame0, c.save_freq, s0, &next_save, &frame);
     // The barrier state as an integer, and the steps of this launch counted in
     // state 1 (StateCount, Single.h:2154-2165; a launch has at most 2^20 steps);
     // s.state, s.cnt0 and s.cnt1 are brought up to date where they are stored.
@@ -1155,7 +1138,7 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         o->n_catch = n_catch;
         o->n_gauss += last - s0;
         if (!f.tau0) o->n_flip += last - s0;
-        o->steps_done = last;
+        o->steps_done = (last == c.run_end) ? c.clock_end : last;  // a finished run's clock skips the dropped steps
         o->error = err;
         lane_param(s, sp, f, false);
         lane_energy_force(s, c, f, false, true);
@@ -1212,8 +1195,8 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
             const bool live = (dead == 0);
             const bool entering = f.model == 0 ? s.ent_period : s.ent_local;
             if (entering && live) {
-                const HandOver r = hand_over_call<MODEL, OVER, TAU0, REPLAY>(
-                    args, t, w, iStep, s.periodic_prev, s.periodic, s.local_prev, s.local, st ? 1.0 : 0.0, n_catch);
+                const HandOver r = hand_over(c, P, f, t, w, iStep, s.periodic_prev, s.periodic, s.local_prev,
+                                             s.local, st ? 1.0 : 0.0, n_catch);
                 n_catch = r.n_catch;
                 err = r.err;
                 if (err) {
@@ -1227,8 +1210,8 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
                 }
             }
             if ((s.ent_period || s.ent_well) && dead == 0) {
-                err = stats_event_call<MODEL, OVER, TAU0, REPLAY>(args, t, w, o, iStep, s.well, s.periodic,
-                                                                  st ? 1.0 : 0.0, s.ent_well, s.ent_period);
+                err = stats_event_at(c, P, f, t, w, o, iStep, s.well, s.periodic, st ? 1.0 : 0.0, s.ent_well,
+                                     s.ent_period);
                 if (err) fail(iStep, iStep - s0);
             }
         }
@@ -1434,7 +1417,7 @@ __global__ void __launch_bounds__(256) run_table_kernel(RunConst c, RunPtrs P, i
     steps[(uint64_t)t * n_ids + k] = step;
 }
 
-using StepKernel = void (*)(RunConst, RunPtrs, uint64_t, uint64_t, const uint4*, const RunArgs*);
+using StepKernel = void (*)(RunConst, RunPtrs, uint64_t, uint64_t, const uint4*);
 
 template <int MODEL, int OVER, int SRC>
 StepKernel step_kernel_tau(const RunConst& c)
@@ -1478,6 +1461,38 @@ uint64_t buffered_steps(uint64_t n, uint64_t nsteps)
 
 }  // namespace ldpcp
 
+namespace ldpcp {
+// A progress line printed from the handle's stream when the run reaches a step
+// (LDPCP_PROGRESS_STEPS=N in the environment: a line every N steps), so that a
+// long run reports what its kernels have done, not what the host enqueued.
+struct ProgressNote {
+    const void* handle;
+    uint32_t traj_offset, n_traj;
+    uint64_t step, end;
+    std::chrono::steady_clock::time_point launched;
+};
+
+static void print_progress(void* p)
+{
+    const ProgressNote* n = (const ProgressNote*)p;
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - n->launched).count();
+    fprintf(stderr, "ldpcp: handle %p trajectories %u-%u reached step %llu of %llu (%.1f %%) after %.1f s\n",
+            n->handle, n->traj_offset, n->traj_offset + n->n_traj - 1, (unsigned long long)n->step,
+            (unsigned long long)n->end, 100.0 * (double)n->step / (double)(n->end ? n->end : 1), sec);
+    fflush(stderr);
+}
+}  // namespace ldpcp
+
+// A copy on the handle's own stream, waited for.  Run mode makes no call on the
+// null stream: a process gets a few hardware queues (GPU_MAX_HW_QUEUES), and a
+// queue taken by the null stream can leave two handles' streams sharing one, which
+// runs their kernels by turns (seen as two of four concurrent ranks at half speed).
+static hipError_t copy_on(hipStream_t s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
+{
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
 struct ldpcp_run {
     ldpcp_run_params params;
     ldpcp::RunConst c;
@@ -1486,15 +1501,15 @@ struct ldpcp_run {
     void* state_block;
     void* frames_block;   // frames of a resumed run (prior frames + this run's), else null
     uint4* noise;         // buffered noise, [buffered_steps(n_traj, nsteps)][n_traj], for handles of few trajectories
-    ldpcp::RunArgs* args; // device copy of c and P for the step kernel, written at each launch
-    ldpcp::RunArgs args_host;
     bool events_complete; // the event lists hold every event since step 0
     uint64_t n_frames;
     uint64_t step_base;   // absolute step the next integrate starts after
+    uint64_t frames_prior; // frames recorded before this handle's run (a resumed image's)
     int phase;            // 0 fresh, 1 resumed (next integrate continues), 2 resumed run done
     bool integrated;
     bool pending;         // launched, not yet waited for
     int device;           // HIP device the handle's memory and stream live on
+    std::vector<ldpcp::ProgressNote> progress;  // notes of the pending run's progress callbacks
 };
 
 using namespace ldpcp;
@@ -1602,13 +1617,11 @@ extern "C" int ldpcp_run_create(const ldpcp_run_params* p, uint32_t n_traj, uint
     // the buffered-noise scratch and the event lists are allocated here, not at
     // launch: an allocation between launches could wait for other handles' runs
     if (n <= kBufferedMaxTraj) e = hipMalloc(&h->noise, buffered_steps(n, h->c.nsteps_eff) * n * sizeof(uint4));
-    if (e == hipSuccess) e = hipMalloc(&h->args, sizeof(RunArgs));
     if (e == hipSuccess && h->c.n_lists)
         e = hipMalloc(&h->P.events, n * (uint64_t)h->c.n_lists * h->c.ev_cap * sizeof(double));
     if (e != hipSuccess) {
         set_error("ldpcp_run_create: hipMalloc of the noise buffer or event lists failed: %s", hipGetErrorString(e));
         if (h->noise) (void)hipFree(h->noise);
-        if (h->args) (void)hipFree(h->args);
         (void)hipStreamDestroy(h->stream);
         (void)hipFree(h->state_block);
         delete h;
@@ -1660,11 +1673,11 @@ extern "C" int ldpcp_run_set_replay(ldpcp_run* h, const double* gauss, uint64_t 
     double* c = (double*)b;
     b += n * catch_cap * sizeof(double);
     int8_t* bi = (int8_t*)b;
-    hipError_t e = hipMemcpy(g, gauss, n_gauss * n * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess && n_flip) e = hipMemcpy(f, flip, n_flip * n * sizeof(double), hipMemcpyHostToDevice);
+    hipError_t e = copy_on(h->stream, g, gauss, n_gauss * n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n_flip) e = copy_on(h->stream, f, flip, n_flip * n * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess && catch_cap)
-        e = hipMemcpy(c, catchup, n * catch_cap * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess && bn) e = hipMemcpy(bi, binit, 2 * n * (uint64_t)bn, hipMemcpyHostToDevice);
+        e = copy_on(h->stream, c, catchup, n * catch_cap * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && bn) e = copy_on(h->stream, bi, binit, 2 * n * (uint64_t)bn, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(block);
         set_error("ldpcp_run_set_replay: copy to the device failed: %s", hipGetErrorString(e));
@@ -1717,16 +1730,20 @@ extern "C" int ldpcp_run_launch(ldpcp_run* h, uint64_t steps_per_launch)
     // steps (step_base, step_base + NSTEPS]: a resumed run continues the absolute step
     // count (Single.h:1420, 1471), which is also the Philox counter of its noise
     const uint64_t end = h->step_base + h->c.nsteps_eff;
+    h->c.run_begin = h->step_base;
+    h->c.run_end = end;
+    h->c.clock_end = h->step_base + h->params.nsteps;
+    h->c.frame0 = h->phase == 0 ? 1 : h->frames_prior;
     const bool buffered = !h->c.replay && n <= kBufferedMaxTraj;
     const int src = h->c.replay ? kNoiseReplay : buffered ? kNoiseBuffered : kNoisePhilox;
     const StepKernel step = step_kernel_for(h->c, src);
     if (buffered) steps_per_launch = std::min(steps_per_launch, buffered_steps(n, h->c.nsteps_eff));
     const dim3 sblock(buffered ? 64 : 256), sgrid((unsigned)((n + sblock.x - 1) / sblock.x));
-    // the step kernel's copy of its arguments (the handle keeps the host side
-    // unchanged while the run is pending)
-    h->args_host.c = h->c;
-    h->args_host.P = h->P;
-    LDPCP_HIP_TRY(hipMemcpyAsync(h->args, &h->args_host, sizeof(RunArgs), hipMemcpyHostToDevice, h->stream));
+    uint64_t every = 0;
+    if (const char* e = getenv("LDPCP_PROGRESS_STEPS")) every = strtoull(e, nullptr, 10);
+    const auto launched = std::chrono::steady_clock::now();
+    h->progress.clear();
+    if (every) h->progress.reserve((end - h->step_base) / every + 2);  // stable addresses for the callbacks
     for (uint64_t s0 = h->step_base; s0 < end; s0 += steps_per_launch) {
         const uint64_t s1 = (s0 + steps_per_launch < end) ? s0 + steps_per_launch : end;
         if (buffered) {
@@ -1734,8 +1751,12 @@ extern "C" int ldpcp_run_launch(ldpcp_run* h, uint64_t steps_per_launch)
             hipLaunchKernelGGL(run_noise_kernel, ngrid, dim3(256), 0, h->stream, h->c, s0, s1 - s0, h->noise);
             LDPCP_HIP_TRY(hipGetLastError());
         }
-        step<<<sgrid, sblock, 0, h->stream>>>(h->c, h->P, s0, s1, h->noise, h->args);
+        step<<<sgrid, sblock, 0, h->stream>>>(h->c, h->P, s0, s1, h->noise);
         LDPCP_HIP_TRY(hipGetLastError());
+        if (every && (s1 / every != s0 / every || s1 == end) && h->progress.size() < h->progress.capacity()) {
+            h->progress.push_back({h, h->c.traj_offset, h->c.n_traj, s1, end, launched});
+            LDPCP_HIP_TRY(hipLaunchHostFunc(h->stream, print_progress, &h->progress.back()));
+        }
     }
     h->pending = true;
     if (h->phase == 1) h->phase = 2;
@@ -1754,7 +1775,7 @@ extern "C" int ldpcp_run_wait(ldpcp_run* h)
     h->integrated = true;
     // surface the first per-lane error
     std::vector<ldpcp_lane> lanes(n);
-    LDPCP_HIP_TRY(hipMemcpy(lanes.data(), h->P.lanes, n * sizeof(ldpcp_lane), hipMemcpyDeviceToHost));
+    LDPCP_HIP_TRY(copy_on(h->stream, lanes.data(), h->P.lanes, n * sizeof(ldpcp_lane), hipMemcpyDeviceToHost));
     for (uint64_t t = 0; t < n; ++t) {
         if (lanes[t].error) {
             const char* what = lanes[t].error == 2   ? "replay barrier table does not cover a touched barrier"
@@ -1783,7 +1804,7 @@ extern "C" int ldpcp_run_lanes(ldpcp_run* h, uint32_t begin, uint32_t n, ldpcp_l
         set_error("ldpcp_run_lanes: bad handle, output or range");
         return LDPCP_EINVAL;
     }
-    LDPCP_HIP_TRY(hipMemcpy(out, h->P.lanes + begin, (uint64_t)n * sizeof(ldpcp_lane), hipMemcpyDeviceToHost));
+    LDPCP_HIP_TRY(copy_on(h->stream, out, h->P.lanes + begin, (uint64_t)n * sizeof(ldpcp_lane), hipMemcpyDeviceToHost));
     std::vector<double> buf((uint64_t)n);
     struct Col {
         const double* src;
@@ -1795,12 +1816,12 @@ extern "C" int ldpcp_run_lanes(ldpcp_run* h, uint32_t begin, uint32_t n, ldpcp_l
                 {h->P.periodic, offsetof(ldpcp_lane, periodic)}, {h->P.xp, offsetof(ldpcp_lane, xp)},
                 {h->P.local, offsetof(ldpcp_lane, local_barrier)}};
     for (const Col& col : cols) {
-        LDPCP_HIP_TRY(hipMemcpy(buf.data(), col.src + begin, (uint64_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        LDPCP_HIP_TRY(copy_on(h->stream, buf.data(), col.src + begin, (uint64_t)n * sizeof(double), hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < n; ++i) memcpy((char*)(out + i) + col.off, &buf[i], sizeof(double));
     }
     std::vector<uint64_t> cnt((uint64_t)n);
     for (int s = 0; s < 2; ++s) {
-        LDPCP_HIP_TRY(hipMemcpy(cnt.data(), (s ? h->P.cnt1 : h->P.cnt0) + begin, (uint64_t)n * sizeof(uint64_t),
+        LDPCP_HIP_TRY(copy_on(h->stream, cnt.data(), (s ? h->P.cnt1 : h->P.cnt0) + begin, (uint64_t)n * sizeof(uint64_t),
                                 hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < n; ++i) out[i].state_count[s] = (double)cnt[i];
     }
@@ -1822,7 +1843,7 @@ extern "C" int ldpcp_run_frames(ldpcp_run* h, int32_t field, uint32_t begin, uin
     }
     const uint64_t nf = h->n_frames, nt = h->c.n_traj;
     std::vector<double> plane(nf * nt);
-    LDPCP_HIP_TRY(hipMemcpy(plane.data(), h->P.frames + (uint64_t)field * nf * nt, nf * nt * sizeof(double),
+    LDPCP_HIP_TRY(copy_on(h->stream, plane.data(), h->P.frames + (uint64_t)field * nf * nt, nf * nt * sizeof(double),
                             hipMemcpyDeviceToHost));
     for (uint32_t i = 0; i < n; ++i)
         for (uint64_t f = 0; f < nf; ++f) out[(uint64_t)i * nf + f] = plane[f * nt + begin + i];
@@ -1839,7 +1860,7 @@ extern "C" int ldpcp_run_ticks(ldpcp_run* h, uint32_t begin, uint32_t n, double*
         set_error("ldpcp_run_ticks: bad handle, range, or no tick capacity");
         return LDPCP_EINVAL;
     }
-    LDPCP_HIP_TRY(hipMemcpy(out, h->P.ticks + (uint64_t)begin * h->c.tick_cap,
+    LDPCP_HIP_TRY(copy_on(h->stream, out, h->P.ticks + (uint64_t)begin * h->c.tick_cap,
                             (uint64_t)n * h->c.tick_cap * sizeof(double), hipMemcpyDeviceToHost));
     return LDPCP_OK;
 }
@@ -1859,13 +1880,13 @@ extern "C" int ldpcp_run_msd(ldpcp_run* h, const uint64_t* lags, uint32_t nlags,
     double* d_out = nullptr;
     LDPCP_HIP_TRY(hipMalloc(&d_lags, nlags * sizeof(uint64_t)));
     LDPCP_HIP_TRY(hipMalloc(&d_out, nt * nlags * sizeof(double)));
-    LDPCP_HIP_TRY(hipMemcpy(d_lags, lags, nlags * sizeof(uint64_t), hipMemcpyHostToDevice));
+    LDPCP_HIP_TRY(copy_on(h->stream, d_lags, lags, nlags * sizeof(uint64_t), hipMemcpyHostToDevice));
     const dim3 block(256), grid((unsigned)((nt + 255) / 256), nlags);
     hipLaunchKernelGGL(msd_kernel, grid, block, 0, h->stream, h->P.frames, h->n_frames, (uint32_t)nt, d_lags, nlags,
                        d_out);
     LDPCP_HIP_TRY(hipGetLastError());
     LDPCP_HIP_TRY(hipStreamSynchronize(h->stream));
-    LDPCP_HIP_TRY(hipMemcpy(out, d_out, nt * nlags * sizeof(double), hipMemcpyDeviceToHost));
+    LDPCP_HIP_TRY(copy_on(h->stream, out, d_out, nt * nlags * sizeof(double), hipMemcpyDeviceToHost));
     (void)hipFree(d_lags);
     (void)hipFree(d_out);
     return LDPCP_OK;
@@ -2007,13 +2028,16 @@ CkptLayout ckpt_layout(uint64_t n, uint64_t wb, uint64_t fields, uint64_t n_fram
 
 uint64_t run_fields(const ldpcp_run* h) { return h->params.save_all ? kFields : 1; }
 
-// absolute step of the state a checkpoint would hold
-uint64_t run_steps_done(const ldpcp_run* h) { return h->phase == 1 ? h->step_base : h->step_base + h->c.nsteps_eff; }
+// the step clock a checkpoint holds: where the next run continues from (the
+// reference's previous + NUMBEROFSTEPS, ldpcp_schedule.h)
+uint64_t run_steps_done(const ldpcp_run* h) { return h->phase == 1 ? h->step_base : h->step_base + h->params.nsteps; }
+
+// frames a checkpoint holds
+uint64_t run_frames_done(const ldpcp_run* h) { return h->phase == 1 ? h->frames_prior : h->n_frames; }
 
 CkptLayout run_layout(const ldpcp_run* h, uint64_t n_events)
 {
-    return ckpt_layout(h->c.n_traj, (uint64_t)h->c.wb, run_fields(h), run_steps_done(h) / h->c.save_freq + 1,
-                       h->c.tick_cap, n_events);
+    return ckpt_layout(h->c.n_traj, (uint64_t)h->c.wb, run_fields(h), run_frames_done(h), h->c.tick_cap, n_events);
 }
 
 }  // namespace
@@ -2033,7 +2057,7 @@ extern "C" int ldpcp_run_set_barrier_default(ldpcp_run* h, const double* state)
             set_error("ldpcp_run_set_barrier_default: trajectory %u: state must be 0 or 1", t);
             return LDPCP_EINVAL;
         }
-    LDPCP_HIP_TRY(hipMemcpy(h->P.bdefault, state, h->c.n_traj * sizeof(double), hipMemcpyHostToDevice));
+    LDPCP_HIP_TRY(copy_on(h->stream, h->P.bdefault, state, h->c.n_traj * sizeof(double), hipMemcpyHostToDevice));
     return LDPCP_OK;
 }
 
@@ -2083,12 +2107,16 @@ extern "C" int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes)
     char* b = (char*)buf;
     ldpcp_lane* lanes = (ldpcp_lane*)(b + L.lanes);
     memcpy(lanes, lv.data(), n * sizeof(ldpcp_lane));
-    for (uint64_t t = 0; t < n; ++t)
+    for (uint64_t t = 0; t < n; ++t) {
         if (lanes[t].error) {
             set_error("ldpcp_run_checkpoint: trajectory %llu stopped with error %d; its state is not resumable",
                       (unsigned long long)t, lanes[t].error);
             return LDPCP_ERANGE;
         }
+        // the clock the state continues from (also when a run of fewer than FREQSAVE
+        // steps integrated none)
+        lanes[t].steps_done = run_steps_done(h);
+    }
     ldpcp_checkpoint_header hd;
     memset(&hd, 0, sizeof(hd));
     hd.magic = LDPCP_CHECKPOINT_MAGIC;
@@ -2102,7 +2130,7 @@ extern "C" int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes)
     hd.n_fields = (int32_t)fields;
     hd.seed = h->params.seed;
     hd.steps_done = run_steps_done(h);
-    hd.n_frames = hd.steps_done / h->c.save_freq + 1;
+    hd.n_frames = run_frames_done(h);
     hd.save_freq = h->c.save_freq;
     hd.tick_capacity = h->c.tick_cap;
     hd.timestep = h->params.timestep;
@@ -2117,7 +2145,7 @@ extern "C" int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes)
     if (n_lists) {
         // the lists, packed: each trajectory's kept entries, list after list
         std::vector<double> all(n * n_lists * ecap);
-        LDPCP_HIP_TRY(hipMemcpy(all.data(), h->P.events, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+        LDPCP_HIP_TRY(copy_on(h->stream, all.data(), h->P.events, all.size() * sizeof(double), hipMemcpyDeviceToHost));
         double* dst = (double*)(b + L.events);
         for (uint64_t t = 0; t < n; ++t)
             for (uint32_t l = 0; l < n_lists; ++l) {
@@ -2126,15 +2154,15 @@ extern "C" int ldpcp_run_checkpoint(ldpcp_run* h, void* buf, uint64_t bytes)
                 dst += keep;
             }
     }
-    LDPCP_HIP_TRY(hipMemcpy(b + L.wbase, h->P.wbase, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-    LDPCP_HIP_TRY(hipMemcpy(b + L.win, h->P.win, n * 2 * wb * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    LDPCP_HIP_TRY(copy_on(h->stream, b + L.wbase, h->P.wbase, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    LDPCP_HIP_TRY(copy_on(h->stream, b + L.win, h->P.win, n * 2 * wb * sizeof(uint64_t), hipMemcpyDeviceToHost));
     // the frames recorded so far: the first n_frames rows of each field's plane
     for (uint64_t f = 0; f < fields; ++f)
-        LDPCP_HIP_TRY(hipMemcpy(b + L.frames + f * hd.n_frames * n * sizeof(double),
+        LDPCP_HIP_TRY(copy_on(h->stream, b + L.frames + f * hd.n_frames * n * sizeof(double),
                                 h->P.frames + f * h->c.frames_cap * n, hd.n_frames * n * sizeof(double),
                                 hipMemcpyDeviceToHost));
     if (h->c.tick_cap)
-        LDPCP_HIP_TRY(hipMemcpy(b + L.ticks, h->P.ticks, n * h->c.tick_cap * sizeof(double), hipMemcpyDeviceToHost));
+        LDPCP_HIP_TRY(copy_on(h->stream, b + L.ticks, h->P.ticks, n * h->c.tick_cap * sizeof(double), hipMemcpyDeviceToHost));
     return LDPCP_OK;
 }
 
@@ -2169,7 +2197,7 @@ extern "C" int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes)
         {hd.seed == h->params.seed, "seed"},
         {hd.save_freq == h->c.save_freq, "save frequency"},
         {hd.timestep == h->params.timestep, "time step"},
-        {hd.n_frames == hd.steps_done / hd.save_freq + 1, "frame count"},
+        {hd.n_frames >= 1 && hd.n_frames <= hd.steps_done / hd.save_freq + 1, "frame count"},
     };
     for (const Check& k : checks)
         if (!k.ok) {
@@ -2208,7 +2236,7 @@ extern "C" int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes)
             return LDPCP_EINVAL;
         }
     // frame buffer for the prior frames and this run's
-    const uint64_t cap = (hd.steps_done + h->c.nsteps_eff) / h->c.save_freq + 1;
+    const uint64_t cap = hd.n_frames + h->c.nsteps_eff / h->c.save_freq;
     void* fb = nullptr;
     LDPCP_HIP_TRY(hipMalloc(&fb, fields * cap * n * sizeof(double)));
     if (h->frames_block) (void)hipFree(h->frames_block);
@@ -2219,10 +2247,10 @@ extern "C" int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes)
     LDPCP_HIP_TRY(hipMemsetAsync(fb, 0, fields * cap * n * sizeof(double), h->stream));
     LDPCP_HIP_TRY(hipStreamSynchronize(h->stream));
     for (uint64_t f = 0; f < fields; ++f)
-        LDPCP_HIP_TRY(hipMemcpy(h->P.frames + f * cap * n, b + L.frames + f * hd.n_frames * n * sizeof(double),
+        LDPCP_HIP_TRY(copy_on(h->stream, h->P.frames + f * cap * n, b + L.frames + f * hd.n_frames * n * sizeof(double),
                                 hd.n_frames * n * sizeof(double), hipMemcpyHostToDevice));
     // per-trajectory state: the statistics record as is, the stepping state into its columns
-    LDPCP_HIP_TRY(hipMemcpy(h->P.lanes, lanes, n * sizeof(ldpcp_lane), hipMemcpyHostToDevice));
+    LDPCP_HIP_TRY(copy_on(h->stream, h->P.lanes, lanes, n * sizeof(ldpcp_lane), hipMemcpyHostToDevice));
     std::vector<double> col(n);
     struct Col {
         double* dst;
@@ -2235,22 +2263,22 @@ extern "C" int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes)
                 {h->P.local, offsetof(ldpcp_lane, local_barrier)}, {h->P.bdefault, offsetof(ldpcp_lane, state0_instance)}};
     for (const Col& c : cols) {
         for (uint64_t t = 0; t < n; ++t) memcpy(&col[t], (const char*)(lanes + t) + c.off, sizeof(double));
-        LDPCP_HIP_TRY(hipMemcpy(c.dst, col.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        LDPCP_HIP_TRY(copy_on(h->stream, c.dst, col.data(), n * sizeof(double), hipMemcpyHostToDevice));
     }
     std::vector<uint64_t> cnt(n);
     for (int k = 0; k < 2; ++k) {
         for (uint64_t t = 0; t < n; ++t) cnt[t] = (uint64_t)lanes[t].state_count[k];
-        LDPCP_HIP_TRY(hipMemcpy(k ? h->P.cnt1 : h->P.cnt0, cnt.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice));
+        LDPCP_HIP_TRY(copy_on(h->stream, k ? h->P.cnt1 : h->P.cnt0, cnt.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice));
     }
-    LDPCP_HIP_TRY(hipMemcpy(h->P.wbase, b + L.wbase, n * sizeof(int32_t), hipMemcpyHostToDevice));
-    LDPCP_HIP_TRY(hipMemcpy(h->P.win, b + L.win, n * 2 * wb * sizeof(uint64_t), hipMemcpyHostToDevice));
+    LDPCP_HIP_TRY(copy_on(h->stream, h->P.wbase, b + L.wbase, n * sizeof(int32_t), hipMemcpyHostToDevice));
+    LDPCP_HIP_TRY(copy_on(h->stream, h->P.win, b + L.win, n * 2 * wb * sizeof(uint64_t), hipMemcpyHostToDevice));
     if (h->c.tick_cap) {
         // barrier ticks beyond the smaller of the two capacities stay counted but unlisted
         std::vector<double> tk(n * h->c.tick_cap, 0.0);
         const uint64_t keep = hd.tick_capacity < h->c.tick_cap ? hd.tick_capacity : h->c.tick_cap;
         for (uint64_t t = 0; t < n; ++t)
             memcpy(&tk[t * h->c.tick_cap], b + L.ticks + t * hd.tick_capacity * sizeof(double), keep * sizeof(double));
-        LDPCP_HIP_TRY(hipMemcpy(h->P.ticks, tk.data(), tk.size() * sizeof(double), hipMemcpyHostToDevice));
+        LDPCP_HIP_TRY(copy_on(h->stream, h->P.ticks, tk.data(), tk.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     // event lists: the image's kept entries, as many as this handle keeps
     const uint64_t ecap = h->c.ev_cap;
@@ -2268,10 +2296,11 @@ extern "C" int ldpcp_run_resume(ldpcp_run* h, const void* buf, uint64_t bytes)
                 memcpy(&all[(t * n_lists + l) * ecap], src, keep * sizeof(double));
                 src += have;
             }
-        LDPCP_HIP_TRY(hipMemcpy(h->P.events, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+        LDPCP_HIP_TRY(copy_on(h->stream, h->P.events, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     h->events_complete = complete;
     h->step_base = hd.steps_done;
+    h->frames_prior = hd.n_frames;
     h->phase = 1;
     h->integrated = false;
     return LDPCP_OK;
@@ -2298,8 +2327,8 @@ extern "C" int ldpcp_run_barrier_table(ldpcp_run* h, int32_t array, int32_t lo, 
                        d_states, d_steps);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e == hipSuccess) e = hipMemcpy(states, d_states, n * ids * sizeof(int32_t), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(steps, d_steps, n * ids * sizeof(uint64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = copy_on(h->stream, states, d_states, n * ids * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = copy_on(h->stream, steps, d_steps, n * ids * sizeof(uint64_t), hipMemcpyDeviceToHost);
     (void)hipFree(block);
     if (e != hipSuccess) {
         set_error("ldpcp_run_barrier_table: %s", hipGetErrorString(e));
@@ -2315,7 +2344,6 @@ extern "C" int ldpcp_run_destroy(ldpcp_run* h)
     if (h->c.replay) (void)hipFree((void*)h->P.g);
     if (h->frames_block) (void)hipFree(h->frames_block);
     if (h->noise) (void)hipFree(h->noise);
-    if (h->args) (void)hipFree(h->args);
     if (h->P.events) (void)hipFree(h->P.events);
     (void)hipFree(h->state_block);
     (void)hipStreamDestroy(h->stream);

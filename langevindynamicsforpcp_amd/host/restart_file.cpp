@@ -240,11 +240,12 @@ bool import_reference_body(Reader& r, FILE* f, const RunnerConfig& c, int W, con
         return true;
     }
 
-    // checks the resumed run relies on
+    // checks the resumed run relies on: at least the step-0 frame, at most one per
+    // FREQSAVE steps of the file's clock (the resumed run records its frames every
+    // FREQSAVE steps from that clock, as the reference's does)
     const uint64_t n_frames = lanes[0].traj[0].size();
-    if (n_frames != nsteps_total / c.save_freq + 1)
-        return fail(f, "the restart file's frames do not match its step count "
-                       "(NUMBEROFSTEPS was not a multiple of FREQSAVE)");
+    if (n_frames < 1 || n_frames > nsteps_total / c.save_freq + 1)
+        return fail(f, "the restart file's frames do not match its step count");
     const int nfields = spec.n_fields;
     const int src_field[7] = {0, 1, 2, 3, 4, 6, 7};  // x v f pe ke state barrier (eta is not kept)
     for (int i = 0; i < W; ++i) {

@@ -235,6 +235,16 @@ void write_mean_log(const std::string& name, const Header& h, const std::vector<
 
 }  // namespace
 
+// Phase times of the integrate and restart path, on stderr when LDPCP_TIMING is set.
+void phase_time(const char* what, int rank, std::chrono::steady_clock::time_point t0,
+                std::chrono::steady_clock::time_point t1)
+{
+    static const bool on = getenv("LDPCP_TIMING") != nullptr;
+    if (!on) return;
+    fprintf(stderr, "ldpcp timing: rank %d %-26s %10.3f s\n", rank, what,
+            std::chrono::duration<double>(t1 - t0).count());
+}
+
 std::vector<int> rank_seeds(int master_seed, int ranks, int* used_master)
 {
     const int value = master_seed == -1 ? (int)time(nullptr) : master_seed;
@@ -388,6 +398,7 @@ int run(const RunnerConfig& cfg)
                     die("ldpcp_run_set_barrier_default");
                 uint64_t nsteps_before = 0;  // m_NumberOfStepDonePreviously
                 if (cfg.read_restart) {
+                    const auto tr0 = clock::now();
                     // PreIntegrate -> ReadRestartFile (Single.h:1346-1351); a failure ends the
                     // run as the runner's integrator-failure check does (:3436-3446)
                     ImageSpec spec;  // what a reference program's file must become for this handle
@@ -399,10 +410,13 @@ int run(const RunnerConfig& cfg)
                     spec.event_capacity = p.event_capacity;
                     spec.n_lists = p.event_capacity ? n_lists : 0;
                     bool ok = read_restart_file(cfg, W, r, i, spec, image, &nsteps_before);
+                    const auto tr1 = clock::now();
                     if (ok && ldpcp_run_resume(h, image.data(), image.size())) {
                         fprintf(stderr, "Error: %s\n", ldpcp_last_error());
                         ok = false;
                     }
+                    phase_time("read restart file", r, tr0, tr1);
+                    phase_time("resume (image to device)", r, tr1, clock::now());
                     if (!ok) {
                         fprintf(stderr, "Error: can't read restart file on rank %10d\n", r);
                         fprintf(stderr, "Error: Integrator failure\n");
@@ -423,7 +437,9 @@ int run(const RunnerConfig& cfg)
             std::vector<std::thread> launchers;
             for (size_t k = 0; k < handles.size(); ++k)
                 launchers.emplace_back([&, k] {
+                    const auto tl0 = clock::now();
                     if (ldpcp_run_launch(handles[k], cfg.steps_per_launch)) launch_error[k] = ldpcp_last_error();
+                    phase_time("launch (enqueue)", (int)(k % R), tl0, clock::now());
                 });
             for (std::thread& th : launchers) th.join();
         }
@@ -432,8 +448,11 @@ int run(const RunnerConfig& cfg)
                 fprintf(stderr, "Error: ldpcp_run_launch: %s\n", msg.c_str());
                 exit(1);
             }
-        for (ldpcp_run* h : handles)
-            if (ldpcp_run_wait(h)) die("ldpcp_run_integrate");
+        for (size_t k = 0; k < handles.size(); ++k) {
+            const auto tw0 = clock::now();
+            if (ldpcp_run_wait(handles[k])) die("ldpcp_run_integrate");
+            phase_time("wait", (int)(k % R), tw0, clock::now());
+        }
         const auto t1 = clock::now();
         t_integrate += std::chrono::duration<double>(t1 - t0).count();
         for (int i = i0; i < i1; ++i) {
@@ -445,6 +464,7 @@ int run(const RunnerConfig& cfg)
                 ldpcp_run* h = handles[(size_t)(i - i0) * R + r];
                 const uint64_t nsteps_before = before[(size_t)(i - i0) * R + r];
                 if (cfg.write_restart) {
+                    const auto tc0 = clock::now();
                     uint64_t bytes = 0;
                     if (ldpcp_run_checkpoint_size(h, &bytes)) die("ldpcp_run_checkpoint_size");
                     image.resize(bytes);
@@ -466,6 +486,7 @@ int run(const RunnerConfig& cfg)
                         ok = write_restart_file(cfg, W, r, i, nsteps_before + cfg.nsteps, image);
                     }
                     if (!ok) fprintf(stderr, "Error: can't open restart file on rank %10d\n", r);
+                    phase_time("checkpoint + restart file", r, tc0, clock::now());
                 }
                 const auto t2 = clock::now();
                 if (ldpcp_run_lanes(h, 0, (uint32_t)W, lanes.data())) die("ldpcp_run_lanes");

@@ -16,6 +16,7 @@ count itself, so:
   trial's initial state (UserData::m_bInitialValue, :1378-1380).
 """
 import ctypes as C
+import math
 
 import numpy as np
 import pytest
@@ -77,6 +78,55 @@ def test_gpu_resume_continues_bit_exact(name):
     assert second.Checkpoint() == full.Checkpoint()
     for le in (full, first, second):
         le.close()
+
+
+def test_gpu_resume_after_dropped_steps_continues_the_reference_clock():
+    """NUMBEROFSTEPS 45 with FREQSAVE 10 integrates 40 steps; the reference's
+    clock, its restart header and its next run continue at step 45 (the resumed
+    run integrates steps 46 .. 75, frames at 55, 65, 75; Single.h:1397-1450).
+    A free overdamped particle (F = 0) with replayed Gaussians has the exact
+    recurrence x <- (x + var g_k) + R, R <- var g_k with k = iStep - 1, so every
+    frame of both runs is known bit for bit."""
+    W = 8
+    p = make("single_over", nsteps=45, save_freq=10)
+    p.height[0][0] = p.height[0][1] = p.height[1][0] = p.height[1][1] = 0.0
+    p.periodic_length, p.randomize_v, p.randomize_state = 1e6, 0, 0
+    p.b_waiting = p.b_fpt = p.b_tpt = p.b_fptcg = 0
+    print("[gpu restart] dropped trailing steps", flush=True)
+    rng = np.random.default_rng(5)
+    n_g = 1 + 45 + 30  # R0, then one Gaussian per step of the clock
+    gauss = rng.standard_normal((n_g, W))
+    flip = rng.random((45 + 30, W))
+    first = to_engine(p, W)
+    first.SetReplay(gauss, flip, np.zeros((W, 1)), np.full((2, W, 1), -1, np.int8), 0)
+    first.Integrate(0)
+    image = first.Checkpoint()
+    hd = N.CheckpointHeader.from_buffer_copy(image)
+    assert hd.steps_done == 45 and hd.n_frames == 5
+    assert all(L.steps_done == 45 for L in first.lanes())
+    p.nsteps = 30
+    second = to_engine(p, W)
+    second.SetReplay(gauss, flip, np.zeros((W, 1)), np.full((2, W, 1), -1, np.int8), 0)
+    second.Resume(image)
+    second.Integrate(0)
+    assert N.load().ldpcp_run_n_frames(second._h) == 8
+    assert all(L.steps_done == 75 for L in second.lanes())
+    # the recurrence, step by step, in the same operation order
+    var = math.sqrt(p.kb * p.temperature / (p.mass * p.gamma) * p.timestep * 0.5)
+    x = np.full(W, p.x0)
+    R = var * gauss[0]
+    want = {0: x.copy()}
+    for i in list(range(1, 41)) + list(range(46, 76)):
+        rf = var * gauss[1 + i - 1]
+        x = (x + rf) + R
+        R = rf
+        if i in (10, 20, 30, 40, 55, 65, 75):
+            want[i] = x.copy()
+    got = second.frames(0)
+    for k, step in enumerate([0, 10, 20, 30, 40, 55, 65, 75]):
+        np.testing.assert_array_equal(got[:, k], want[step], err_msg=f"frame {k} (step {step})")
+    first.close()
+    second.close()
 
 
 def test_gpu_checkpoint_of_a_resumed_handle_is_the_image():
