@@ -290,11 +290,19 @@ typedef struct ldpcp_set_stats {
     double tptb_sum[2], tptb_sum2[2];
     uint64_t traj_steps;           /* trajectory-steps integrated */
     uint64_t n_finished;           /* absorbed + capped so far */
-    double hist_lo, hist_hi;       /* log10 edges of the FPT histogram, in time units */
+    /* FPT histogram: 256 log10 bins over [hist_lo, hist_hi] (log10 of time);
+     * absorption step k (FPT = dt k) falls in bin
+     * min(255, floor((log10(dt k) - hist_lo) * 256 / (hist_hi - hist_lo))) */
+    double hist_lo, hist_hi;
     uint64_t hist[LDPCP_HIST_BINS];
     uint64_t msd_step[LDPCP_MSD_LAGS]; /* lag steps (log-spaced in [1, max_steps]) */
     double msd_sum[LDPCP_MSD_LAGS];    /* sum of (x(lag) - x0)^2 over trajectories alive at the lag */
     uint64_t msd_n[LDPCP_MSD_LAGS];
+    /* first absorption step of each histogram bin, 0 where no step up to the
+     * cap reaches the bin: bin b holds the steps k with hist_step[b] <= k and
+     * k < hist_step[b'] for the next b' > b with hist_step[b'] > hist_step[b]
+     * (bins too narrow to hold an integer step repeat the next start) */
+    uint64_t hist_step[LDPCP_HIST_BINS];
 } ldpcp_set_stats;
 
 /* Launch shape.  Zero fields take the defaults (block 256, 8 blocks per CU,

@@ -97,6 +97,7 @@ class SetStats(C.Structure):
         ("msd_step", C.c_uint64 * MSD_LAGS),
         ("msd_sum", C.c_double * MSD_LAGS),
         ("msd_n", C.c_uint64 * MSD_LAGS),
+        ("hist_step", C.c_uint64 * HIST_BINS),
     ]
 
 

@@ -47,7 +47,6 @@
 namespace ldpcp {
 
 constexpr int kMaxBlock = 256;
-constexpr int kWavesPerBlock = kMaxBlock / 64;
 constexpr uint32_t kNoLag = 0xffffffffu;
 
 // per-set constants (FP64 and FP32 copies of IntegrateInitializeVariables,
@@ -69,6 +68,7 @@ struct SetDev {
     int32_t slot_base, well0;                       // first of the two cells (slot 0)
     uint64_t n_traj, max_steps, first, start;       // first slot; engine step at admission
     double hist_lo, hist_inv;                       // log10 histogram edges
+    uint32_t hist_edge[LDPCP_HIST_BINS];            // first absorption step of each bin (0xffffffff: past the cap)
     uint32_t lag[LDPCP_MSD_LAGS];                   // MSD lag steps, ascending, kNoLag past the cap
 };
 
@@ -222,36 +222,74 @@ __device__ __forceinline__ int lag_index(const SetDev* Sg, uint32_t s)
     return lo;
 }
 
+// A lane's first passage in this launch window (step 0: none).
+struct Absorb {
+    uint32_t step, tpt0, xf, xl;
+};
+
+// Bytes of LDS per wavefront in engine_step: MSD partial sums and counts (one
+// lag per lane) and the wavefront's private FPT histogram.
+constexpr size_t kLdsPerWave = LDPCP_MSD_LAGS * 8 + LDPCP_MSD_LAGS * 4 + LDPCP_HIST_BINS * 4;
+
+// The first passages of one wavefront's lanes of one set (lanes of other sets
+// and lanes without one pass step 0): the eight FPT/TPT sums are reduced
+// across the wavefront and added once by the set's leading lane; each passage
+// is counted in the wavefront's LDS histogram, whose non-zero bins are then
+// added to the set's histogram.  The bin of absorption step s is the last k
+// with hist_edge[k] <= s (integer edges, so no logarithm on the device).
 template <int MODEL>
-__device__ __forceinline__ void record_absorption(const SetDev* __restrict__ Sg, const SlotPtrs& P,
-                                                  const StatsPtrs& St, int set, uint32_t slot, uint32_t step,
-                                                  uint32_t tpt0, uint32_t xf, uint32_t xl)
+__device__ __forceinline__ void flush_absorptions(const SetDev* __restrict__ Sg, const StatsPtrs& St, int set,
+                                                  const Absorb a, uint64_t absorbed_mask, int lane, int leader,
+                                                  uint32_t* __restrict__ hist)
 {
-    const SetDev& S = *Sg;
-    P.absorbed[slot] = step;
-    P.tpt0[slot] = tpt0;
+    const double dt = uni(Sg->dt);
+    const bool on = a.step != 0u;
+    // one sum at a time, each total moved to scalar registers, so that the
+    // reduction holds two VGPRs rather than sixteen
+    auto wave_total = [&](double x) __attribute__((always_inline)) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        return uni(x);
+    };
     double* sm = St.sums + (uint64_t)set * kSumsPerSet;
-    const double fpt = S.dt * (double)step;
-    const double tpt = S.dt * (double)(step - tpt0);
-    atomicAdd(sm + 0, fpt);
-    atomicAdd(sm + 1, fpt * fpt);
-    atomicAdd(sm + 2, tpt);
-    atomicAdd(sm + 3, tpt * tpt);
+    auto add = [&](int i, double x) __attribute__((always_inline)) {
+        const double t = wave_total(x);
+        if (lane == leader) atomicAdd(sm + i, t);
+    };
+    const double fpt = on ? dt * (double)a.step : 0.0;
+    add(0, fpt);
+    add(1, fpt * fpt);
+    const double tpt = on ? dt * (double)(a.step - a.tpt0) : 0.0;
+    add(2, tpt);
+    add(3, tpt * tpt);
     if (MODEL == 1) {
-        P.xc_first[slot] = xf;
-        P.xc_last[slot] = xl;
-        const double b0 = xf ? S.dt * (double)(xf - tpt0) : 0.0;
-        const double b1 = S.dt * (double)(step - xl);
-        atomicAdd(sm + 4, b0);
-        atomicAdd(sm + 5, b0 * b0);
-        atomicAdd(sm + 6, b1);
-        atomicAdd(sm + 7, b1 * b1);
+        const double b0 = (on && a.xf) ? dt * (double)(a.xf - a.tpt0) : 0.0;
+        add(4, b0);
+        add(5, b0 * b0);
+        const double b1 = on ? dt * (double)(a.step - a.xl) : 0.0;
+        add(6, b1);
+        add(7, b1 * b1);
     }
-    atomicAdd(St.cnt + (uint64_t)set * kCntPerSet + 0, 1ull);
-    // bin on log10(fpt) = log2(fpt) * log10(2), in FP32 (only the bin index is kept)
-    int bin = (int)((__builtin_amdgcn_logf((float)fpt) * 0.30102999566f - (float)S.hist_lo) * (float)S.hist_inv);
-    bin = bin < 0 ? 0 : (bin >= LDPCP_HIST_BINS ? LDPCP_HIST_BINS - 1 : bin);
-    atomicAdd(St.hist + (uint64_t)set * LDPCP_HIST_BINS + bin, 1ull);
+    if (lane == leader)
+        atomicAdd(St.cnt + (uint64_t)set * kCntPerSet + 0, (unsigned long long)__popcll(absorbed_mask));
+    if (on) {
+        int b = 0;
+#pragma unroll
+        for (int h = LDPCP_HIST_BINS / 2; h > 0; h >>= 1)
+            if (Sg->hist_edge[b + h] <= a.step) b += h;
+        atomicAdd(hist + b, 1u);
+    }
+    __threadfence_block();
+#pragma unroll
+    for (int k = 0; k < LDPCP_HIST_BINS / 64; ++k) {
+        const int b = lane + 64 * k;
+        const uint32_t n = hist[b];
+        if (n) {
+            atomicAdd(St.hist + (uint64_t)set * LDPCP_HIST_BINS + b, (unsigned long long)n);
+            hist[b] = 0u;
+        }
+    }
+    __threadfence_block();
 }
 
 // catch-up flip on entering a barrier (langevinDynamicsSingleBarrier.h:2039-2051):
@@ -352,7 +390,8 @@ __device__ __forceinline__ float force_f32(const Hot& H, float xp, int state)
 template <int MODEL, bool OVER, bool EQ>
 __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St, const Hot H,
                                          const SetDev* __restrict__ Sg, int set, uint32_t slot, const Window W,
-                                         double* lds_sum, uint32_t* lds_n, bool& fin, uint32_t& nsteps)
+                                         double* lds_sum, uint32_t* lds_n, bool& fin, uint32_t& nsteps,
+                                         Absorb& ab)
 {
     const uint32_t tid = slot - H.first;
     float xp = P.xp[slot], comp = P.comp[slot], v = P.fv[slot], R = P.fR[slot];
@@ -464,7 +503,6 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
         const bool hit =
             (MODEL == 0) ? ent_well : (ent_period && (unsigned)(cell - H.slot_base) > 1u);
         if (hit) {
-            record_absorption<MODEL>(Sg, P, St, set, slot, step, tpt0, xf, xl);
             done = true;
             last = step;
             break;
@@ -474,6 +512,8 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
             if (MODEL == 1) xf = 0;
         }
     }
+    if (done) P.absorbed[slot] = last;
+    ab.step = done ? last : 0u;
     if (!done && W.s_end >= H.max_steps) done = true;  // capped
     if (done) flags &= ~F_ALIVE;
     flags = (flags & ~F_STATE) | (uint32_t)state;
@@ -492,6 +532,9 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
     }
     fin = done;
     nsteps = last - W.s0;
+    ab.tpt0 = tpt0;
+    ab.xf = xf;
+    ab.xl = xl;
 }
 
 // ---------------------------------------------------------------------------
@@ -521,7 +564,8 @@ __device__ __forceinline__ double force_f64(const Hot& H, double x, double perio
 template <int MODEL, bool OVER, bool EQ>
 __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St, const Hot H,
                                          const SetDev* __restrict__ Sg, int set, uint32_t slot, const Window W,
-                                         double* lds_sum, uint32_t* lds_n, bool& fin, uint32_t& nsteps)
+                                         double* lds_sum, uint32_t* lds_n, bool& fin, uint32_t& nsteps,
+                                         Absorb& ab)
 {
     const uint32_t tid = slot - H.first;
     double x = P.x[slot], v = P.v[slot], R = P.R[slot];
@@ -616,7 +660,6 @@ __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St,
         const int kcell = (int)periodic - H.slot_base;
         const bool hit = (MODEL == 0) ? ent_well : (ent_period && (kcell < 0 || kcell > 1));
         if (hit) {
-            record_absorption<MODEL>(Sg, P, St, set, slot, step, tpt0, xf, xl);
             done = true;
             last = step;
             break;
@@ -626,6 +669,8 @@ __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St,
             if (MODEL == 1) xf = 0;
         }
     }
+    if (done) P.absorbed[slot] = last;
+    ab.step = done ? last : 0u;
     if (!done && W.s_end >= H.max_steps) done = true;
     if (done) flags &= ~F_ALIVE;
     flags = (flags & ~F_STATE) | (uint32_t)state;
@@ -642,22 +687,25 @@ __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St,
     }
     fin = done;
     nsteps = last - W.s0;
+    ab.tpt0 = tpt0;
+    ab.xf = xf;
+    ab.xl = xl;
 }
 
 template <bool F32, int MODEL, bool OVER>
 __device__ __forceinline__ void body(const SlotPtrs& P, const StatsPtrs& St, const SetDev* __restrict__ Sg, int set,
                                      uint32_t slot, const Window W, double* lds_sum, uint32_t* lds_n, bool& fin,
-                                     uint32_t& nsteps)
+                                     uint32_t& nsteps, Absorb& ab)
 {
     const bool eq = __builtin_amdgcn_readfirstlane(Sg->equal_prob) != 0;
     if (F32) {
         const Hot H = load_hot<true, MODEL, OVER>(Sg);
-        if (eq) body_f32<MODEL, OVER, true>(P, St, H, Sg, set, slot, W, lds_sum, lds_n, fin, nsteps);
-        else body_f32<MODEL, OVER, false>(P, St, H, Sg, set, slot, W, lds_sum, lds_n, fin, nsteps);
+        if (eq) body_f32<MODEL, OVER, true>(P, St, H, Sg, set, slot, W, lds_sum, lds_n, fin, nsteps, ab);
+        else body_f32<MODEL, OVER, false>(P, St, H, Sg, set, slot, W, lds_sum, lds_n, fin, nsteps, ab);
     } else {
         const Hot H = load_hot<false, MODEL, OVER>(Sg);
-        if (eq) body_f64<MODEL, OVER, true>(P, St, H, Sg, set, slot, W, lds_sum, lds_n, fin, nsteps);
-        else body_f64<MODEL, OVER, false>(P, St, H, Sg, set, slot, W, lds_sum, lds_n, fin, nsteps);
+        if (eq) body_f64<MODEL, OVER, true>(P, St, H, Sg, set, slot, W, lds_sum, lds_n, fin, nsteps, ab);
+        else body_f64<MODEL, OVER, false>(P, St, H, Sg, set, slot, W, lds_sum, lds_n, fin, nsteps, ab);
     }
 }
 
@@ -667,18 +715,28 @@ __device__ __forceinline__ void body(const SlotPtrs& P, const StatsPtrs& St, con
 
 // One launch: persistent workgroups; each wavefront takes 64-entry chunks of
 // the live list until none is left, and records the chunk's survivors as a
-// 64-bit mask for the compaction.
+// 64-bit mask for the compaction.  The FP32 kernel is held to 96 VGPRs (five
+// wavefronts per SIMD); the step loop needs 93, the absorption flush after it
+// would otherwise take a few more.
 template <bool F32>
-__global__ void __launch_bounds__(kMaxBlock) engine_step(SlotPtrs P, StatsPtrs St, const SetDev* __restrict__ sets,
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(F32 ? 5 : 1))) engine_step(SlotPtrs P, StatsPtrs St, const SetDev* __restrict__ sets,
                                                          Ctrl* ctrl, const uint32_t* __restrict__ live,
                                                          unsigned long long* __restrict__ mask, int parity, uint32_t K)
 {
-    __shared__ double s_sum[kWavesPerBlock][LDPCP_MSD_LAGS];
-    __shared__ uint32_t s_n[kWavesPerBlock][LDPCP_MSD_LAGS];
+    // dynamic LDS, kLdsPerWave bytes per wavefront: [waves][64] MSD sums,
+    // [waves][64] MSD counts, [waves][256] FPT histogram bins
+    extern __shared__ __align__(16) unsigned char lds_raw[];
     const int wv = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    s_sum[wv][lane] = 0.0;
-    s_n[wv][lane] = 0u;
+    const int n_waves = blockDim.x >> 6;
+    double* const s_sum = (double*)lds_raw + wv * LDPCP_MSD_LAGS;
+    uint32_t* const s_n = (uint32_t*)(lds_raw + (size_t)n_waves * LDPCP_MSD_LAGS * 8) + wv * LDPCP_MSD_LAGS;
+    uint32_t* const s_hist =
+        (uint32_t*)(lds_raw + (size_t)n_waves * LDPCP_MSD_LAGS * 12) + wv * LDPCP_HIST_BINS;
+    s_sum[lane] = 0.0;
+    s_n[lane] = 0u;
+#pragma unroll
+    for (int k = 0; k < LDPCP_HIST_BINS / 64; ++k) s_hist[lane + 64 * k] = 0u;
     __syncthreads();
     const uint32_t n_live = ctrl->n_live[parity];
     const unsigned long long estep = ctrl->estep;
@@ -711,13 +769,20 @@ __global__ void __launch_bounds__(kMaxBlock) engine_step(SlotPtrs P, StatsPtrs S
             W.j0 = lag_index(Sg, W.s0);
             bool fin = false;
             uint32_t nst = 0;
+            Absorb ab = {0u, 0u, 0u, 0u};
+            const int kind = __builtin_amdgcn_readfirstlane(Sg->kind);
             if (mine) {
-                switch (Sg->kind) {
-                case 0: body<F32, 0, false>(P, St, Sg, s, slot, W, s_sum[wv], s_n[wv], fin, nst); break;
-                case 1: body<F32, 0, true>(P, St, Sg, s, slot, W, s_sum[wv], s_n[wv], fin, nst); break;
-                case 2: body<F32, 1, false>(P, St, Sg, s, slot, W, s_sum[wv], s_n[wv], fin, nst); break;
-                default: body<F32, 1, true>(P, St, Sg, s, slot, W, s_sum[wv], s_n[wv], fin, nst); break;
+                switch (kind) {
+                case 0: body<F32, 0, false>(P, St, Sg, s, slot, W, s_sum, s_n, fin, nst, ab); break;
+                case 1: body<F32, 0, true>(P, St, Sg, s, slot, W, s_sum, s_n, fin, nst, ab); break;
+                case 2: body<F32, 1, false>(P, St, Sg, s, slot, W, s_sum, s_n, fin, nst, ab); break;
+                default: body<F32, 1, true>(P, St, Sg, s, slot, W, s_sum, s_n, fin, nst, ab); break;
                 }
+            }
+            const uint64_t abm = __ballot(ab.step != 0u);
+            if (abm) {
+                if (kind < 2) flush_absorptions<0>(Sg, St, s, ab, abm, lane, leader, s_hist);
+                else flush_absorptions<1>(Sg, St, s, ab, abm, lane, leader, s_hist);
             }
             const uint64_t finm = __ballot(fin);
             alive |= group & ~finm;
@@ -734,12 +799,12 @@ __global__ void __launch_bounds__(kMaxBlock) engine_step(SlotPtrs P, StatsPtrs S
             __threadfence_block();
             const int jl = W.j0 + lane;
             if (jl < j1) {
-                const uint32_t n = s_n[wv][jl];
+                const uint32_t n = s_n[jl];
                 if (n) {
-                    atomicAdd(St.msd_sum + (uint64_t)s * LDPCP_MSD_LAGS + jl, s_sum[wv][jl]);
+                    atomicAdd(St.msd_sum + (uint64_t)s * LDPCP_MSD_LAGS + jl, s_sum[jl]);
                     atomicAdd(St.msd_n + (uint64_t)s * LDPCP_MSD_LAGS + jl, (unsigned long long)n);
-                    s_sum[wv][jl] = 0.0;
-                    s_n[wv][jl] = 0u;
+                    s_sum[jl] = 0.0;
+                    s_n[jl] = 0u;
                 }
             }
             __threadfence_block();
@@ -1079,6 +1144,29 @@ static void fill_set_dev(const ldpcp_sweep_set& s, uint64_t first, uint64_t star
     d.hist_lo = log10(s.timestep);
     const double hi = log10(s.timestep * (double)s.max_steps) + 1e-9;
     d.hist_inv = LDPCP_HIST_BINS / (hi - d.hist_lo);
+    // bin of absorption step k: floor((log10(dt k) - lo) * inv), clamped to
+    // [0, 255]; it is non-decreasing in k, so each bin is a step range and the
+    // device compares the integer step against the range starts
+    const double lo = d.hist_lo, inv = d.hist_inv;
+    auto bin_of = [&](uint64_t k) {
+        const double b = floor((log10(s.timestep * (double)k) - lo) * inv);
+        return b < 0.0 ? 0 : (b > LDPCP_HIST_BINS - 1 ? LDPCP_HIST_BINS - 1 : (int)b);
+    };
+    d.hist_edge[0] = 0;
+    const int top = bin_of(s.max_steps);
+    for (int b = 1; b < LDPCP_HIST_BINS; ++b) {
+        if (b > top) {
+            d.hist_edge[b] = 0xffffffffu;
+            continue;
+        }
+        uint64_t a = 1, z = s.max_steps;  // smallest k in [1, max_steps] with bin_of(k) >= b
+        while (a < z) {
+            const uint64_t m = a + (z - a) / 2;
+            if (bin_of(m) >= b) z = m;
+            else a = m + 1;
+        }
+        d.hist_edge[b] = (uint32_t)a;
+    }
     // 64 log-spaced lag steps in [1, max_steps], strictly increasing
     uint32_t prev = 0;
     for (int k = 0; k < LDPCP_MSD_LAGS; ++k) {
@@ -1264,11 +1352,12 @@ static unsigned scatter_grid(int n_cu) { return (unsigned)std::max(1, n_cu * 4);
 int Engine::launch_plain(uint32_t K)
 {
     const dim3 grid((unsigned)(n_cu_ * cfg_.blocks_per_cu)), block((unsigned)cfg_.block);
+    const size_t lds = (size_t)(cfg_.block / 64) * kLdsPerWave;
     if (precision_ == 32)
-        hipLaunchKernelGGL(engine_step<true>, grid, block, 0, stream_, P_, St_, (const SetDev*)d_sets_, d_ctrl_,
+        hipLaunchKernelGGL(engine_step<true>, grid, block, lds, stream_, P_, St_, (const SetDev*)d_sets_, d_ctrl_,
                            (const uint32_t*)d_live_[parity_], d_mask_, parity_, K);
     else
-        hipLaunchKernelGGL(engine_step<false>, grid, block, 0, stream_, P_, St_, (const SetDev*)d_sets_, d_ctrl_,
+        hipLaunchKernelGGL(engine_step<false>, grid, block, lds, stream_, P_, St_, (const SetDev*)d_sets_, d_ctrl_,
                            (const uint32_t*)d_live_[parity_], d_mask_, parity_, K);
     LDPCP_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(engine_scan, dim3(1), dim3(1024), 0, stream_, d_ctrl_, (const unsigned long long*)d_mask_,
@@ -1437,6 +1526,8 @@ int Engine::stats(int id, ldpcp_set_stats* out, hipStream_t s)
     out->hist_lo = d.hist_lo;
     out->hist_hi = d.hist_lo + LDPCP_HIST_BINS / d.hist_inv;
     for (int k = 0; k < LDPCP_MSD_LAGS; ++k) out->msd_step[k] = d.lag[k] == kNoLag ? 0 : d.lag[k];
+    out->hist_step[0] = 1;
+    for (int k = 1; k < LDPCP_HIST_BINS; ++k) out->hist_step[k] = d.hist_edge[k] == 0xffffffffu ? 0 : d.hist_edge[k];
     return LDPCP_OK;
 }
 

@@ -49,6 +49,14 @@ struct ldpcp_server {
     int pending_slot = -1;       // counter buffer written by a launch not yet read
     int pending_parity = 0;      // live-list parity after that launch
     std::vector<uint64_t> steps_of;  // engine set id -> trajectory-steps at the last read
+    // A set id freed by a retirement can be given to a new request before the
+    // counters of the launch in flight are read; those counters still hold the
+    // previous owner's values.  admitted_at[id] is the index of the first launch
+    // that runs the id's current request, counter_launch[k] the index of the
+    // launch whose counters buffer k holds; process() skips ids admitted after it.
+    std::vector<uint64_t> admitted_at;
+    uint64_t n_launched = 0;
+    uint64_t counter_launch[2] = {0, 0};
     hipStream_t copy_stream = nullptr;
 
     ~ldpcp_server()
@@ -72,6 +80,7 @@ struct ldpcp_server {
             if (id < 0) break;  // no contiguous slot range or set id free yet
             r.state = LDPCP_REQ_RUNNING;
             r.engine_id = id;
+            admitted_at[id] = n_launched;
             resident_bytes += r.bytes;
             by_engine_id[id] = queue.front();
             queue.pop_front();
@@ -89,6 +98,7 @@ struct ldpcp_server {
         for (int id = 0; id < eng.max_sets(); ++id) {
             const uint64_t rid = by_engine_id[id];
             if (!rid) continue;
+            if (admitted_at[id] > counter_launch[k]) continue;  // counters predate this request
             Request& r = reqs[rid];
             steps_of[id] = cnt[(uint64_t)id * kCntPerSet + 2];
             if (cnt[(uint64_t)id * kCntPerSet + 1] < r.set.n_traj) continue;
@@ -143,6 +153,7 @@ extern "C" int ldpcp_server_create(const ldpcp_server_config* cfg, ldpcp_server*
     s->budget = cfg->budget_bytes;
     s->by_engine_id.assign((size_t)max_sets, 0);
     s->steps_of.assign((size_t)max_sets, 0);
+    s->admitted_at.assign((size_t)max_sets, 0);
     *out = s;
     return LDPCP_OK;
 }
@@ -202,6 +213,7 @@ extern "C" int ldpcp_server_pump(ldpcp_server* s, uint32_t n_launches, int32_t w
         rc = s->eng.launch(0);
         if (rc) return rc;
         const int k = s->slot;
+        s->counter_launch[k] = s->n_launched++;
         rc = s->eng.enqueue_counters(k);
         if (rc) return rc;
         const int parity_now = s->eng.parity();

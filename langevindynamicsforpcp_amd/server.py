@@ -83,7 +83,7 @@ class Server:
 
 
 def poisson_requests(rate, duration, seed, n_min=1000, n_max=1000000, eb_lo=2.0, eb_hi=6.5, ratio=3.6,
-                     max_steps=50000, timestep=1e-3, tau=1e-4):
+                     max_steps=100000, timestep=1e-3, tau=1e-4):
     """Timed sweep requests: Poisson arrivals at `rate` per second over
     `duration` seconds; each request draws single or double barrier, under- or
     overdamped, Eb ~ U[eb_lo, eb_hi] (heights ratio*Eb / Eb) and a trajectory
