@@ -87,6 +87,40 @@ double apex_x(const RunnerConfig& c)
     return fma(x1h * y1, x2b - x2l, x2l * y2 * (x1h - x1b)) / fma(y1, x2b - x2l, y2 * (x1h - x1b));
 }
 
+// The double barrier's per-state potential constants a lane carries in its
+// restart record (IntegrateInitializeVariables, Double.h:1845-1899, selected by
+// UpdateBarrierStateEtaParamter, :2737-2753): [barrier][state][0 falling side
+// (m_SlopeNegative / m_InterceptNegative), 1 rising side (…Positive)].  The
+// programs recompute them at every step, so they are written for the
+// reference's reader and ignored on reading.
+struct DoubleBarrierConstants {
+    double slope[2][2][2], intercept[2][2][2];
+};
+
+DoubleBarrierConstants double_barrier_constants(const RunnerConfig& c)
+{
+    const double x1b = c.position[0], w1 = c.half_width[0], x2b = c.position[1], w2 = c.half_width[1];
+    const double xc = apex_x(c);
+    const double yc = (x1b + w1 - xc) / (x1b + w1 - x1b) * c.height[0][0];  // GetXc, Double.h:1730
+    const double low0 = x1b - w1, high1 = x2b + w2;
+    const double xb_xl0 = x1b - low0, xh_xb1 = high1 - x2b;
+    DoubleBarrierConstants k;
+    for (int s = 0; s < 2; ++s) {
+        const double h0 = c.height[0][s], h1 = c.height[1][s];
+        k.slope[0][s][0] = h0 / xb_xl0;
+        k.slope[0][s][1] = (yc - h0) / (xc - x1b);
+        k.slope[1][s][0] = (h1 - yc) / (x2b - xc);
+        k.slope[1][s][1] = 0.0 - h1 / xh_xb1;
+        k.intercept[0][s][0] = 0.0 - h0 * (low0 / xb_xl0);
+        k.intercept[0][s][1] = (xc * h0 - yc * x1b) / (xc - x1b);
+        k.intercept[1][s][0] = (yc * x2b - xc * h1) / (x2b - xc);
+        k.intercept[1][s][1] = h1 * (high1 / xh_xb1);
+    }
+    return k;
+}
+
+constexpr int kDoubleTablesEnd = 9948;  // after the double barrier's two tables (Double.h:640)
+
 // ReadRestartFile's error exit: message, close, false (Single.h:1143-1190)
 bool fail(FILE* f, const char* what)
 {
@@ -96,11 +130,14 @@ bool fail(FILE* f, const char* what)
 }
 
 // One lane of TrajectoryData as WriteRestartFile lays it out
-// (langevinDynamicsSingleBarrier.h:436-572).
+// (langevinDynamicsSingleBarrier.h:436-572; langevinDynamicsDoubleBarrier.h:465-634,
+// which adds xp, the per-barrier slopes, intercepts and heights and the local
+// barrier indices).
 struct RefLane {
-    double x, v, f, e, eta, barrier;
+    double x, v, f, e, eta, barrier, xp = 0.0;
     std::vector<double> traj[8];  // x, v, f, pe, ke, eta, state, barrier frames
-    double state, prev_state, slope_pos, slope_neg, height, count0, count1, well, periodic, well_prev, periodic_prev;
+    double state, prev_state, count0, count1, well, periodic, well_prev, periodic_prev;
+    double local = 0.0;  // m_LocalBarrierIndex (double barrier)
     double v0_instance, eta0_instance, state0_instance;
     double tick_prev = 0.0;
     int passed = 0;
@@ -118,19 +155,20 @@ struct RefLane {
 
 bool read_ref_lane(Reader& r, const RunnerConfig& c, bool wt, bool fpt, RefLane& L)
 {
+    const bool dbl = c.model != 0;
     L.x = r.f64();
     L.v = r.f64();
     L.f = r.f64();
     L.e = r.f64();
     L.eta = r.f64();
     L.barrier = r.f64();
+    if (dbl) L.xp = r.f64();
     for (auto& t : L.traj)
         if (!r.vec(t)) return false;
     L.state = r.f64();
     L.prev_state = r.f64();
-    L.slope_pos = r.f64();
-    L.slope_neg = r.f64();
-    L.height = r.f64();
+    // slopes (and intercepts) and the current height(s): recomputed by the next step
+    for (int k = 0; k < (dbl ? 10 : 3); ++k) r.f64();
     L.count0 = r.f64();
     L.count1 = r.f64();
     L.well = r.f64();
@@ -139,6 +177,14 @@ bool read_ref_lane(Reader& r, const RunnerConfig& c, bool wt, bool fpt, RefLane&
     L.periodic_prev = r.f64();
     r.i32();  // entering flags: recomputed by the next step
     r.i32();
+    if (dbl) {
+        L.local = r.f64();
+        r.f64();  // previous local barrier index, its entering flag and the *Stored copies:
+        r.i32();  // recomputed by the next step
+        r.f64();
+        r.f64();
+        r.i32();
+    }
     L.v0_instance = r.f64();
     L.eta0_instance = r.f64();
     L.state0_instance = r.f64();
@@ -201,7 +247,30 @@ double list_sum(const std::vector<double>& v)
     return s;
 }
 
-// The reference program's single-barrier body -> checkpoint image.
+// A barrier table as the reference's file holds it ([id][lane]).
+struct RefTable {
+    uint64_t n = 0;
+    int min_id = 0;
+    std::vector<int32_t> state;
+    std::vector<uint64_t> step;
+};
+
+bool read_ref_table(Reader& r, int W, RefTable& t)
+{
+    t.n = r.u64();
+    if (r.eof() || t.n > (1u << 24)) return false;
+    t.state.resize(t.n * W);
+    t.step.resize(t.n * W);
+    for (uint64_t b = 0; b < t.n; ++b)
+        for (int i = 0; i < W; ++i) {
+            t.state[b * W + i] = r.i32();
+            t.step[b * W + i] = r.u64();
+        }
+    t.min_id = r.i32();
+    return !r.eof();
+}
+
+// The reference program's body (either model) -> checkpoint image.
 bool import_reference_body(Reader& r, FILE* f, const RunnerConfig& c, int W, const ImageSpec& spec,
                            uint64_t nsteps_total, std::vector<char>& image)
 {
@@ -213,16 +282,13 @@ bool import_reference_body(Reader& r, FILE* f, const RunnerConfig& c, int W, con
     std::vector<RefLane> lanes(W);
     for (int i = 0; i < W; ++i)
         if (!read_ref_lane(r, c, wt, fpt, lanes[i])) return fail(f, "restart file corrupted");
-    const uint64_t nb = r.u64();
-    if (r.eof() || nb > (1u << 24)) return fail(f, "restart file corrupted");
-    std::vector<int32_t> bstate(nb * W);
-    std::vector<uint64_t> bstep(nb * W);
-    for (uint64_t b = 0; b < nb; ++b)
-        for (int i = 0; i < W; ++i) {
-            bstate[b * W + i] = r.i32();
-            bstep[b * W + i] = r.u64();
-        }
-    const int min_id = r.i32();
+    const int n_arrays = c.model != 0 ? 2 : 1;
+    RefTable tables[2];
+    for (int a = 0; a < n_arrays; ++a)
+        if (!read_ref_table(r, W, tables[a])) return fail(f, "restart file corrupted");
+    // the double barrier's tables end with their own marker, which the
+    // reference's reader only reports (Double.h:890-894)
+    if (c.model != 0 && r.i32() != kDoubleTablesEnd) return fail(f, "wrong signiture");
     std::vector<double> r_prev(W, 0.0);
     if (c.overdamped)
         for (auto& v : r_prev) v = r.f64();
@@ -330,6 +396,8 @@ bool import_reference_body(Reader& r, FILE* f, const RunnerConfig& c, int W, con
         L.r_prev = r_prev[i];
         L.state = R.state;
         L.barrier = R.barrier;
+        L.xp = R.xp;
+        L.local_barrier = R.local;
         L.state_count[0] = R.count0;
         L.state_count[1] = R.count1;
         L.well = R.well;
@@ -383,14 +451,17 @@ bool import_reference_body(Reader& r, FILE* f, const RunnerConfig& c, int W, con
         L.fpt_init_index[0] = R.fpt_init_index[0];
         L.fpt_init_index[1] = R.fpt_init_index[1];
         memcpy(b + o_lanes + i * sizeof(ldpcp_lane), &L, sizeof(L));
-        // barrier table -> the lane's window, centred on its current barrier
+        // barrier tables -> the lane's window, centred on its current barrier
         const int32_t base = (int32_t)R.periodic - spec.barrier_window / 2;
         memcpy(b + o_wbase + i * sizeof(int32_t), &base, sizeof(base));
-        for (uint64_t k = 0; k < nb; ++k) {
-            const int64_t slot = (int64_t)min_id + (int64_t)k - base;
-            if (slot < 0 || slot >= (int64_t)wb) continue;  // beyond reach of the window
-            const uint64_t e = kValid | (bstate[k * W + i] ? kState : 0ull) | (bstep[k * W + i] & kStepMask);
-            memcpy(b + o_win + (((uint64_t)i * 2) * wb + (uint64_t)slot) * sizeof(uint64_t), &e, sizeof(e));
+        for (int a = 0; a < n_arrays; ++a) {
+            const RefTable& T = tables[a];
+            for (uint64_t k = 0; k < T.n; ++k) {
+                const int64_t slot = (int64_t)T.min_id + (int64_t)k - base;
+                if (slot < 0 || slot >= (int64_t)wb) continue;  // beyond reach of the window
+                const uint64_t e = kValid | (T.state[k * W + i] ? kState : 0ull) | (T.step[k * W + i] & kStepMask);
+                memcpy(b + o_win + (((uint64_t)i * 2 + a) * wb + (uint64_t)slot) * sizeof(uint64_t), &e, sizeof(e));
+            }
         }
         for (int fld = 0; fld < nfields; ++fld)
             for (uint64_t fr = 0; fr < n_frames; ++fr)
@@ -511,7 +582,7 @@ bool write_restart_file(const RunnerConfig& c, int simd_width, int rank, int tri
 
 bool reference_layout_possible(const RunnerConfig& c, const std::vector<char>& image)
 {
-    if (c.model != 0 || image.size() < sizeof(ldpcp_checkpoint_header)) return false;
+    if (image.size() < sizeof(ldpcp_checkpoint_header)) return false;
     ldpcp_checkpoint_header hd;
     memcpy(&hd, image.data(), sizeof(hd));
     if (!hd.events_complete) return false;
@@ -521,7 +592,9 @@ bool reference_layout_possible(const RunnerConfig& c, const std::vector<char>& i
     return true;
 }
 
-void barrier_span(const std::vector<char>& image, int* lo, int* hi)
+int step0_barrier_reach(const RunnerConfig& c) { return c.model != 0 ? 10000 : 100; }
+
+void barrier_span(const std::vector<char>& image, int array, int* lo, int* hi)
 {
     ldpcp_checkpoint_header hd;
     memcpy(&hd, image.data(), sizeof(hd));
@@ -533,7 +606,7 @@ void barrier_span(const std::vector<char>& image, int* lo, int* hi)
         memcpy(&base, b + t * sizeof(int32_t), sizeof(base));
         for (uint64_t k = 0; k < wb; ++k) {
             uint64_t e;
-            memcpy(&e, win + ((t * 2) * wb + k) * sizeof(uint64_t), sizeof(e));
+            memcpy(&e, win + ((t * 2 + (uint64_t)array) * wb + k) * sizeof(uint64_t), sizeof(e));
             if (!(e >> 63)) continue;  // never created
             const int id = base + (int)k;
             if (id < *lo) *lo = id;
@@ -543,9 +616,13 @@ void barrier_span(const std::vector<char>& image, int* lo, int* hi)
 }
 
 bool write_reference_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, uint64_t nsteps_total,
-                                  const std::vector<char>& image, int table_lo, int table_n,
-                                  const std::vector<int32_t>& states, const std::vector<uint64_t>& steps)
+                                  const std::vector<char>& image, const std::vector<BarrierTable>& tables)
 {
+    const bool dbl = c.model != 0;
+    if (tables.size() != (dbl ? 2u : 1u)) {
+        fprintf(stderr, "Error: restart file: %zu barrier tables for a model with %d\n", tables.size(), dbl ? 2 : 1);
+        return false;
+    }
     ldpcp_checkpoint_header hd;
     memcpy(&hd, image.data(), sizeof(hd));
     const uint64_t n = hd.n_traj, wb = (uint64_t)hd.barrier_window, nf = hd.n_frames, cap = hd.tick_capacity;
@@ -580,7 +657,8 @@ bool write_reference_restart_file(const RunnerConfig& c, int simd_width, int ran
     Writer w(f);
     const SaveFlags sf = save_flags(c);
     write_header(w, c, kReferenceSignature, simd_width, rank, nsteps_total);
-    // TrajectoryData::WriteRestartFile (Single.h:433-572)
+    // TrajectoryData::WriteRestartFile (Single.h:433-572; Double.h:470-640)
+    const DoubleBarrierConstants dk = dbl ? double_barrier_constants(c) : DoubleBarrierConstants{};
     w.i32(kBodyStart);
     w.i32(c.waiting ? 1 : 0);
     w.i32(c.fpt ? 1 : 0);
@@ -595,13 +673,13 @@ bool write_reference_restart_file(const RunnerConfig& c, int simd_width, int ran
     for (uint64_t t = 0; t < n; ++t) {
         const ldpcp_lane& L = lanes[t];
         const int st = L.state == 1.0 ? 1 : 0;
-        const double height = c.height[0][st], slope = c.height[0][st] / c.half_width[0];
         w.f64(L.x);
         w.f64(L.v);
         w.f64(L.f);
         w.f64(L.e);
         w.f64(0.0);  // eta: the programs have no eta noise on
         w.f64(L.barrier);
+        if (dbl) w.f64(L.xp);
         for (int k = 0; k < 8; ++k) {
             const int fld = field_of[k];
             if (fld < 0 || fld >= hd.n_fields) {
@@ -613,9 +691,20 @@ bool write_reference_restart_file(const RunnerConfig& c, int simd_width, int ran
         }
         w.f64(L.state);
         w.f64(L.state);  // previous state: overwritten by the next step before any use
-        w.f64(slope);
-        w.f64(0.0 - slope);
-        w.f64(height);
+        if (!dbl) {
+            // UpdateBarrierStateEtaParamter (Single.h:2154-2165)
+            const double slope = c.height[0][st] / c.half_width[0];
+            w.f64(slope);
+            w.f64(0.0 - slope);
+            w.f64(c.height[0][st]);
+        } else {
+            for (int side = 1; side >= 0; --side)  // m_SlopePositive[0..1], then m_SlopeNegative[0..1]
+                for (int k = 0; k < 2; ++k) w.f64(dk.slope[k][st][side]);
+            for (int side = 1; side >= 0; --side)  // m_InterceptPositive, m_InterceptNegative
+                for (int k = 0; k < 2; ++k) w.f64(dk.intercept[k][st][side]);
+            w.f64(c.height[0][st]);
+            w.f64(c.height[1][st]);
+        }
         w.f64(L.state_count[0]);
         w.f64(L.state_count[1]);
         w.f64(L.well);
@@ -624,6 +713,15 @@ bool write_reference_restart_file(const RunnerConfig& c, int simd_width, int ran
         w.f64(L.periodic);
         w.i32(0);
         w.i32(0);
+        if (dbl) {
+            // local barrier index, previous (overwritten by the next step), entering
+            // flag, and the same three as the *Stored copies (Double.h:558-563)
+            for (int k = 0; k < 2; ++k) {
+                w.f64(L.local_barrier);
+                w.f64(L.local_barrier);
+                w.i32(0);
+            }
+        }
         w.f64(L.v0_instance);
         w.f64(0.0);
         w.f64(L.state0_instance);
@@ -675,14 +773,18 @@ bool write_reference_restart_file(const RunnerConfig& c, int simd_width, int ran
             }
         }
     }
-    // the barrier table (DataLocator.h:163-169), then R_prev (:2776-2785)
-    w.u64((uint64_t)table_n);
-    for (int k = 0; k < table_n; ++k)
-        for (uint64_t t = 0; t < n; ++t) {
-            w.i32(states[t * table_n + k]);
-            w.u64(steps[t * table_n + k]);
-        }
-    w.i32(table_lo);
+    // the barrier table(s) (DataLocator.h:163-169; the double barrier's two and
+    // their marker, Double.h:637-640), then R_prev (Single.h:2776-2785, Double.h:3391-3400)
+    for (const BarrierTable& T : tables) {
+        w.u64((uint64_t)T.n);
+        for (int k = 0; k < T.n; ++k)
+            for (uint64_t t = 0; t < n; ++t) {
+                w.i32(T.states[t * T.n + k]);
+                w.u64(T.steps[t * T.n + k]);
+            }
+        w.i32(T.lo);
+    }
+    if (dbl) w.i32(kDoubleTablesEnd);
     if (c.overdamped)
         for (uint64_t t = 0; t < n; ++t) w.f64(lanes[t].r_prev);
     w.i32(kRestartEnd);
@@ -709,8 +811,6 @@ bool read_restart_file(const RunnerConfig& c, int simd_width, int rank, int tria
     const int signature = r.i32();
     const bool reference_file = (signature == kReferenceSignature);
     if (signature != kRestartSignature && !reference_file) return fail(f, "inconsistent signature");
-    if (reference_file && c.model != 0)
-        return fail(f, "restart files of the reference's double-barrier program are not read by this build");
     if (r.i32() != simd_width) return fail(f, "inconsistent simd witdth");
     if (r.f64() != c.x0) return fail(f, "inconsistent x0");
     if (r.f64() != c.v0) return fail(f, "inconsistent v0");

@@ -43,34 +43,49 @@ struct ImageSpec {
 };
 
 // Whether the image can be written in the reference program's own layout:
-// single barrier, event lists complete (every event since step 0 kept), all
-// waiting-time ticks kept.
+// event lists complete (every event since step 0 kept), all waiting-time ticks
+// kept.
 bool reference_layout_possible(const RunnerConfig& c, const std::vector<char>& image);
 
-// Widens [*lo, *hi] to every barrier id of array 0 a trajectory of the image
-// holds in its window.
-void barrier_span(const std::vector<char>& image, int* lo, int* hi);
+// Widens [*lo, *hi] to every barrier id of `array` (0, or 1 for the double
+// barrier's second barrier) a trajectory of the image holds in its window.
+void barrier_span(const std::vector<char>& image, int array, int* lo, int* hi);
 
-// WriteRestartFile in the reference program's own layout (signature 2345,
-// TrajectoryData::WriteRestartFile, langevinDynamicsSingleBarrier.h:385-576, the
-// barrier table of ids table_lo .. table_lo + table_n - 1 from
-// ldpcp_run_barrier_table, the overdamped R_prev), readable by the reference
-// program, followed after its end marker by this program's checkpoint image,
-// which read_restart_file prefers (exact continuation).
+// The ids the reference creates in every barrier array at step 0, which its
+// restart files therefore always list: -100 .. 100 for the single barrier
+// (langevinDynamicsSingleBarrier.h:1898-1903), -10000 .. 10000 in both arrays
+// of the double barrier (langevinDynamicsDoubleBarrier.h:2250-2255).
+int step0_barrier_reach(const RunnerConfig& c);
+
+// One barrier array of the reference's restart file (FiniteInifiniteIntegerTickArray1D,
+// DataLocator.h:163-169): ids lo .. lo + n - 1, states and steps as
+// ldpcp_run_barrier_table returns them ([trajectory][id]).
+struct BarrierTable {
+    int lo = 0, n = 0;
+    std::vector<int32_t> states;
+    std::vector<uint64_t> steps;
+};
+
+// WriteRestartFile in the reference program's own layout (signature 2345):
+// TrajectoryData::WriteRestartFile (langevinDynamicsSingleBarrier.h:385-576;
+// langevinDynamicsDoubleBarrier.h:414-642 with its two barrier tables and the
+// marker 9948), the overdamped R_prev (Single.h:2776-2785, Double.h:3391-3400),
+// readable by the reference program, followed after its end marker by this
+// program's checkpoint image, which read_restart_file prefers (exact
+// continuation).  tables holds one table per barrier array of the model.
 bool write_reference_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, uint64_t nsteps_total,
-                                  const std::vector<char>& image, int table_lo, int table_n,
-                                  const std::vector<int32_t>& states, const std::vector<uint64_t>& steps);
+                                  const std::vector<char>& image, const std::vector<BarrierTable>& tables);
 
 // ReadRestartFile: false (after the reference's message on stderr) when the
 // file is missing or any setting differs from c.  On success image holds the
 // checkpoint image and nsteps_total the header's step count.  Besides this
-// program's files it reads the reference program's own single-barrier files
+// program's files it reads the reference program's own files of either model
 // (signature 2345; body TrajectoryData::WriteRestartFile,
-// langevinDynamicsSingleBarrier.h:385-576, then the barrier table, DataLocator.h:163-169,
-// and the overdamped R_prev, :2776-2785): their per-lane state, event lists
-// (as running sums and counts), frames and barrier table become a checkpoint
-// image for `spec`, so a reference run continues here (with this program's
-// Philox noise from the file's step count on).
+// langevinDynamicsSingleBarrier.h:385-576 / langevinDynamicsDoubleBarrier.h:414-642,
+// then the barrier table(s), DataLocator.h:163-169, and the overdamped R_prev):
+// their per-lane state, event lists (as running sums and counts), frames and
+// barrier tables become a checkpoint image for `spec`, so a reference run
+// continues here (with this program's Philox noise from the file's step count on).
 bool read_restart_file(const RunnerConfig& c, int simd_width, int rank, int trial, const ImageSpec& spec,
                        std::vector<char>& image, uint64_t* nsteps_total);
 

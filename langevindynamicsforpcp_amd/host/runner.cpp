@@ -471,17 +471,21 @@ int run(const RunnerConfig& cfg)
                     if (ldpcp_run_checkpoint(h, image.data(), bytes)) die("ldpcp_run_checkpoint");
                     bool ok;
                     if (reference_layout_possible(cfg, image)) {
-                        // the barrier table spans every barrier a trajectory touched and the
-                        // -100 .. 100 the reference creates at step 0 (Single.h:1898-1903)
-                        int lo = -100, hi = 100;
-                        barrier_span(image, &lo, &hi);
-                        const int nb = hi - lo + 1;
-                        std::vector<int32_t> st((size_t)W * nb);
-                        std::vector<uint64_t> sp((size_t)W * nb);
-                        if (ldpcp_run_barrier_table(h, 0, lo, hi, st.data(), sp.data()))
-                            die("ldpcp_run_barrier_table");
-                        ok = write_reference_restart_file(cfg, W, r, i, nsteps_before + cfg.nsteps, image, lo, nb, st,
-                                                          sp);
+                        // each barrier table spans every barrier a trajectory touched and the
+                        // ids the reference creates at step 0
+                        std::vector<BarrierTable> tables(cfg.model != 0 ? 2 : 1);
+                        for (size_t a = 0; a < tables.size(); ++a) {
+                            BarrierTable& T = tables[a];
+                            int lo = -step0_barrier_reach(cfg), hi = step0_barrier_reach(cfg);
+                            barrier_span(image, (int)a, &lo, &hi);
+                            T.lo = lo;
+                            T.n = hi - lo + 1;
+                            T.states.resize((size_t)W * T.n);
+                            T.steps.resize((size_t)W * T.n);
+                            if (ldpcp_run_barrier_table(h, (int32_t)a, lo, hi, T.states.data(), T.steps.data()))
+                                die("ldpcp_run_barrier_table");
+                        }
+                        ok = write_reference_restart_file(cfg, W, r, i, nsteps_before + cfg.nsteps, image, tables);
                     } else {
                         ok = write_restart_file(cfg, W, r, i, nsteps_before + cfg.nsteps, image);
                     }

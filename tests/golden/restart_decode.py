@@ -2,8 +2,16 @@
 
 Layout follows langevinDynamicsSingleBarrier.h:1048-1136 (header),
 :385-576 (TrajectoryData::WriteRestartFile), DataLocator.h:163-169
-(barrier table) and :2776-2785 (overdamped R_prev tail).  All values are
-little-endian: int = 4 B, size_t / long long = 8 B, double = 8 B.
+(barrier table) and :2776-2785 (overdamped R_prev tail).  The double-barrier
+program (model=1) writes langevinDynamicsDoubleBarrier.h:1202-1294 (header
+with four heights, the barrier positions and xc) and :414-642 (per lane also
+xp, two slopes, intercepts and heights each, the local-barrier indices; then
+two barrier tables and the marker 9948), R_prev after it (:3391-3400).  All
+values are little-endian: int = 4 B, size_t / long long = 8 B, double = 8 B.
+
+The single-barrier layout is pinned by the reference's recorded quick-test
+files, which decode and re-encode byte for byte; no double-barrier restart
+file ships with the reference, so model=1 follows the source alone.
 """
 import struct
 import numpy as np
@@ -74,11 +82,34 @@ def read_header(r, model=0):
     return h
 
 
-def decode(path, overdamped=True, tpt=True, fptcg=True):
-    """Decode one restart file into a dict of header fields and per-lane data."""
+# per-lane scalars between the frame lists and the state counts
+_PARAMS = {0: ("current_state", "previous_state", "slope_pos", "slope_neg", "current_height"),
+           1: ("current_state", "previous_state", "slope_pos0", "slope_pos1", "slope_neg0", "slope_neg1",
+               "intercept_pos0", "intercept_pos1", "intercept_neg0", "intercept_neg1",
+               "current_height0", "current_height1")}
+_LEAD = {0: ("x", "v", "f", "e", "eta", "barrier"), 1: ("x", "v", "f", "e", "eta", "barrier", "xp")}
+_LOCAL = (("local", "f64"), ("local_prev", "f64"), ("entering_local", "i32"),
+          ("local_stored", "f64"), ("local_prev_stored", "f64"), ("entering_local_stored", "i32"))
+
+
+def _read_table(r, W):
+    nb = r.u64()
+    states = np.zeros((nb, W), np.int32)
+    steps = np.zeros((nb, W), np.uint64)
+    for b in range(nb):
+        for i in range(W):
+            states[b, i] = r.i32()
+            steps[b, i] = r.u64()
+    return states, steps, r.i32()
+
+
+def decode(path, overdamped=True, tpt=True, fptcg=True, model=0):
+    """Decode one restart file into a dict of header fields and per-lane data.
+    For the double barrier (model=1) the second barrier table is returned as
+    barrier_state1 / barrier_step1 / barrier_min_id1."""
     with open(path, "rb") as f:
         r = _Reader(f.read())
-    h = read_header(r)
+    h = read_header(r, model)
     W = h["simd_width"]
     assert r.i32() == 9998
     b_wt = r.i32()
@@ -86,15 +117,17 @@ def decode(path, overdamped=True, tpt=True, fptcg=True):
     lanes = []
     for _ in range(W):
         L = {}
-        for k in ("x", "v", "f", "e", "eta", "barrier"):
+        for k in _LEAD[model]:
             L[k] = r.f64()
         for k in ("traj_x", "traj_v", "traj_f", "traj_pe", "traj_ke", "traj_eta", "traj_state", "traj_barrier"):
             L[k] = r.sized_vec()
-        for k in ("current_state", "previous_state", "slope_pos", "slope_neg", "current_height",
-                  "state_count0", "state_count1", "well", "periodic", "well_prev", "periodic_prev"):
+        for k in _PARAMS[model] + ("state_count0", "state_count1", "well", "periodic", "well_prev", "periodic_prev"):
             L[k] = r.f64()
         L["entering_well"] = r.i32()
         L["entering_period"] = r.i32()
+        if model == 1:
+            for k, kind in _LOCAL:
+                L[k] = getattr(r, kind)()
         for k in ("v0_instance", "eta0_instance", "state0_instance", "well_stored", "periodic_stored",
                   "well_prev_stored", "periodic_prev_stored"):
             L[k] = r.f64()
@@ -132,14 +165,12 @@ def decode(path, overdamped=True, tpt=True, fptcg=True):
                 lst = r.sized_vec()
                 L["cg"].append({"list": lst, "last_tick": r.f64(), "to_update": r.i32(), "initial_well": r.i32()})
         lanes.append(L)
-    nb = r.u64()
-    states = np.zeros((nb, W), np.int32)
-    steps = np.zeros((nb, W), np.uint64)
-    for b in range(nb):
-        for i in range(W):
-            states[b, i] = r.i32()
-            steps[b, i] = r.u64()
-    min_id = r.i32()
+    states, steps, min_id = _read_table(r, W)
+    extra = {}
+    if model == 1:
+        s1, p1, m1 = _read_table(r, W)
+        extra = {"barrier_state1": s1, "barrier_step1": p1, "barrier_min_id1": m1}
+        assert r.i32() == 9948
     tail = None
     if overdamped:
         tail = r.vec(W)
@@ -153,7 +184,7 @@ def decode(path, overdamped=True, tpt=True, fptcg=True):
         r.pos += n
     assert r.pos == len(r.buf), (r.pos, len(r.buf))
     return {"header": h, "lanes": lanes, "barrier_state": states, "barrier_step": steps,
-            "barrier_min_id": min_id, "r_prev": tail, "trailer": trailer}
+            "barrier_min_id": min_id, "r_prev": tail, "trailer": trailer, **extra}
 
 
 class _Writer:
@@ -180,11 +211,20 @@ class _Writer:
         self.vec(a)
 
 
-def encode(d, path, overdamped=True, tpt=True, fptcg=True):
-    """The inverse of decode for the single-barrier layout: writes the dict
-    decode returns (header, lanes, barrier table, R_prev) as a restart file of
-    the reference program, for tests that need such a file on a machine
-    without the reference's own files."""
+def _write_table(w, states, steps, min_id):
+    w.u64(states.shape[0])
+    for b in range(states.shape[0]):
+        for i in range(states.shape[1]):
+            w.i32(states[b, i])
+            w.u64(steps[b, i])
+    w.i32(min_id)
+
+
+def encode(d, path, overdamped=True, tpt=True, fptcg=True, model=0):
+    """The inverse of decode: writes the dict decode returns (header, lanes,
+    barrier table(s), R_prev) as a restart file of the reference program, for
+    tests that need such a file on a machine without the reference's own
+    files."""
     w = _Writer()
     h = d["header"]
     w.i32(h["signature"])
@@ -199,7 +239,11 @@ def encode(d, path, overdamped=True, tpt=True, fptcg=True):
         w.i32(v)
     w.i32(h["randomize_v"])
     w.i32(h["randomize_eta"])
-    for k in ("periodic_length", "h0", "h1", "initial_state"):
+    if model == 0:
+        keys = ("periodic_length", "h0", "h1", "initial_state")
+    else:
+        keys = ("periodic_length", "h00", "h01", "h10", "h11", "initial_state", "position0", "position1", "xc")
+    for k in keys:
         w.f64(h[k])
     w.i32(h["randomize_state"])
     w.i32(h["same_initial_barrier"])
@@ -217,15 +261,17 @@ def encode(d, path, overdamped=True, tpt=True, fptcg=True):
     w.i32(b_wt)
     w.i32(b_fpt)
     for L in d["lanes"]:
-        for k in ("x", "v", "f", "e", "eta", "barrier"):
+        for k in _LEAD[model]:
             w.f64(L[k])
         for k in ("traj_x", "traj_v", "traj_f", "traj_pe", "traj_ke", "traj_eta", "traj_state", "traj_barrier"):
             w.sized_vec(L[k])
-        for k in ("current_state", "previous_state", "slope_pos", "slope_neg", "current_height",
-                  "state_count0", "state_count1", "well", "periodic", "well_prev", "periodic_prev"):
+        for k in _PARAMS[model] + ("state_count0", "state_count1", "well", "periodic", "well_prev", "periodic_prev"):
             w.f64(L[k])
         w.i32(L["entering_well"])
         w.i32(L["entering_period"])
+        if model == 1:
+            for k, kind in _LOCAL:
+                getattr(w, kind)(L[k])
         for k in ("v0_instance", "eta0_instance", "state0_instance", "well_stored", "periodic_stored",
                   "well_prev_stored", "periodic_prev_stored"):
             w.f64(L[k])
@@ -266,13 +312,10 @@ def encode(d, path, overdamped=True, tpt=True, fptcg=True):
                 w.f64(cg["last_tick"])
                 w.i32(cg["to_update"])
                 w.i32(cg["initial_well"])
-    states, steps = d["barrier_state"], d["barrier_step"]
-    w.u64(states.shape[0])
-    for b in range(states.shape[0]):
-        for i in range(states.shape[1]):
-            w.i32(states[b, i])
-            w.u64(steps[b, i])
-    w.i32(d["barrier_min_id"])
+    _write_table(w, d["barrier_state"], d["barrier_step"], d["barrier_min_id"])
+    if model == 1:
+        _write_table(w, d["barrier_state1"], d["barrier_step1"], d["barrier_min_id1"])
+        w.i32(9948)
     if overdamped:
         w.vec(d["r_prev"])
     w.i32(9990)

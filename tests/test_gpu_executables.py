@@ -516,3 +516,94 @@ def test_executable_writes_the_reference_restart_layout(tmp_path):
     for k, v in before.items():
         np.testing.assert_array_equal(np.fromfile(tmp_path / f"out_{k}.dat", "<f8"), v, err_msg=k)
     np.testing.assert_array_equal(np.loadtxt(tmp_path / "out_msd.log", comments="#"), msd_before)
+
+
+@pytest.mark.parametrize("over", ["true", "false"])
+def test_double_barrier_executable_writes_the_reference_restart_layout(tmp_path, over):
+    """WRITERESTART of the double-barrier program writes the reference's own
+    layout (TrajectoryData::WriteRestartFile, langevinDynamicsDoubleBarrier.h:414-642):
+    per lane xp = x - periodic*L and the local-barrier index (xp < xc -> 0,
+    Double.h:2198-2224) besides the single barrier's record, two barrier
+    tables spanning at least the ids -10000 .. 10000 the reference creates at
+    step 0 (:2250-2255), the marker 9948, R_prev when overdamped.  Every
+    lane's FPT2 and FPTCG lists average to the .dat values the run wrote and
+    its x frames end at its position.  Cut after the end marker, the file
+    alone restarts the run: with NUMBEROFSTEPS 0 the outputs are the writing
+    run's; 100000 further steps add exactly that many steps to the state
+    counts."""
+    import sys
+    sys.path.insert(0, GOLD)
+    import restart_decode as R
+    base = DOUBLE_INP.format(over=over)
+    for k, v in dict(WRITERESTART="true", OUTPUTRESTART="rs", NUMBEROFRUNS="2").items():
+        base = re.sub(rf"(?m)^{k} .*$", f"{k} {v}", base)
+    ranks, W, trials = 2, 4, 2
+    overdamped = over == "true"
+    exe = os.path.join(BIN, "langevinDynamicsDoubleBarrier")
+
+    def run(text, what):
+        (tmp_path / "run.inp").write_text(text)
+        print(f"[gpu exe double reference layout] over={over} {what}", flush=True)
+        r = subprocess.run([exe, "-i", "run.inp", "--ranks", str(ranks), "--simd-width", str(W)], cwd=tmp_path,
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0 and "Error" not in r.stderr, r.stderr[-2000:]
+
+    run(base, "write")
+    names = ("firstPassageTime2", "firstPassageTimeCG2_0000_0002", "firstPassageTimeCG2_0001_0003")
+    dat = {k: np.fromfile(tmp_path / f"dbl_{k}.dat", "<f8") for k in names}
+
+    def mean(v):
+        s = 0.0
+        for x in v:
+            s += x
+        return s / len(v) if len(v) else 0.0
+
+    counted = 0.0
+    n_fpt2 = 0
+    for rk in range(ranks):
+        for tr in range(trials):
+            d = R.decode(str(tmp_path / f"rs_{rk:05d}_{tr:05d}.restart"), overdamped=overdamped, model=1)
+            h = d["header"]
+            assert (h["signature"], h["simd_width"], h["rank"], h["nsteps_total"]) == (2345, W, rk, 200000)
+            assert (h["h00"], h["h01"], h["h10"], h["h11"], h["position0"], h["position1"]) == \
+                (1.0, 0.5, 2.0, 0.5, -1.42, -0.56)
+            assert d["trailer"] is not None
+            xc = h["xc"]
+            for ch, L in enumerate(d["lanes"]):
+                g = rk * trials * W + tr * W + ch      # rank-major, then trial, then lane
+                assert mean(L["fpt2"]) == dat["firstPassageTime2"][g]
+                assert mean(L["cg"][0]["list"]) == dat["firstPassageTimeCG2_0000_0002"][g]
+                assert mean(L["cg"][1]["list"]) == dat["firstPassageTimeCG2_0001_0003"][g]
+                n_fpt2 += len(L["fpt2"])
+                assert L["xp"] == L["x"] - L["periodic"] * 5.8
+                assert L["local"] == (0.0 if L["xp"] < xc else 1.0) == L["local_stored"]
+                assert len(L["traj_x"]) == 200000 // 100 + 1 and L["traj_x"][-1] == L["x"]
+                assert L["state_count0"] + L["state_count1"] == 200001
+                s = int(L["current_state"])
+                assert (L["current_height0"], L["current_height1"]) == ((1.0, 0.5)[s], (2.0, 0.5)[s])
+                counted += L["state_count0"] + L["state_count1"]
+            for a in ("", "1"):
+                st = d["barrier_state" + a]
+                assert d["barrier_min_id" + a] <= -10000 and st.shape[0] >= 20001
+                assert set(np.unique(st)) <= {0, 1}
+            assert (d["r_prev"] is not None) == overdamped
+    assert n_fpt2 > 0, "no FPT2 events: the case does not exercise the lists"
+    # the reference layout alone: cut every file after its end marker
+    for f in os.listdir(tmp_path):
+        if f.endswith(".restart"):
+            d = R.decode(str(tmp_path / f), overdamped=overdamped, model=1)
+            data = (tmp_path / f).read_bytes()
+            (tmp_path / f).write_bytes(data[:len(data) - len(d["trailer"]) - 16])
+            assert R.decode(str(tmp_path / f), overdamped=overdamped, model=1)["trailer"] is None
+    msd_before = np.loadtxt(tmp_path / "dbl_msd.log", comments="#")
+    resumed = base
+    for k, v in dict(WRITERESTART="false", READRESTART="true", INPUTRESTART="rs", NUMBEROFSTEPS="0",
+                     NUMBEROFPREVIOUSSTEPS="200000").items():
+        resumed = re.sub(rf"(?m)^{k} .*$", f"{k} {v}", resumed)
+    run(resumed, "read, 0 steps")
+    for k, v in dat.items():
+        np.testing.assert_array_equal(np.fromfile(tmp_path / f"dbl_{k}.dat", "<f8"), v, err_msg=k)
+    np.testing.assert_array_equal(np.loadtxt(tmp_path / "dbl_msd.log", comments="#"), msd_before)
+    run(re.sub(r"(?m)^NUMBEROFSTEPS .*$", "NUMBEROFSTEPS 100000", resumed), "read, 100000 steps")
+    sc = [ln for ln in (tmp_path / "dbl_stateCount.log").read_text().splitlines() if ln.startswith("# state")]
+    assert float(sc[0].split()[3]) + float(sc[1].split()[3]) == counted + ranks * trials * W * 100000

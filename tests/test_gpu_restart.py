@@ -217,11 +217,12 @@ def test_gpu_same_initial_barrier_second_trial_matches_oracle(oracle):
     le.close()
 
 
-@pytest.mark.parametrize("name", ["single_over", "single_under", "single_eq_false"])
+@pytest.mark.parametrize("name", ["single_over", "single_under", "single_eq_false", "double_over", "double_under"])
 def test_gpu_event_lists_match_oracle(oracle, name):
     """The raw event lists a restart file in the reference's layout needs
     (FPT1, FPT2, TPT1, TPT2, FPTCG and the FPT1 barrier logs of the crossed and
-    the left barrier, TrajectoryData, langevinDynamicsSingleBarrier.h:160-235),
+    the left barrier, TrajectoryData, langevinDynamicsSingleBarrier.h:160-235;
+    the double barrier fills FPT2 and FPTCG, langevinDynamicsDoubleBarrier.h:2931-3209),
     kept on the device, equal the oracle's lists entry for entry (replayed noise)."""
     W, capn = 8, 8192
     p = make(name, nsteps=60000, save_freq=100)
@@ -246,5 +247,8 @@ def test_gpu_event_lists_match_oracle(oracle, name):
             np.testing.assert_array_equal(got[k], cap[key][i, :len(got[k])], err_msg=f"{key} lane {i}")
         for j in range(p.n_cg):
             np.testing.assert_array_equal(got[6 + j], cap["cg"][j, i, :len(got[6 + j])], err_msg=f"cg{j}")
-    assert sum(len(img["events"][i][4]) for i in range(W)) > 0, "no FPT1 event: the barrier logs are not exercised"
+    if name.startswith("single"):
+        assert sum(len(img["events"][i][4]) for i in range(W)) > 0, "no FPT1 event: the barrier logs are not exercised"
+    else:
+        assert sum(len(img["events"][i][1]) for i in range(W)) > 0, "no FPT2 event: the lists are not exercised"
     le.close()

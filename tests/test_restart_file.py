@@ -23,18 +23,25 @@ HARNESS = r"""
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include "ldpcp.h"
 #include "main_common.h"
 #include "restart_file.h"
 using namespace ldpcp_host;
-// import <inp> <restart base> <rank> <W> <image out>: the configuration of the
-// input file, the restart file of (rank, trial 0) as a checkpoint image
-static int import_mode(char** argv)
+static RunnerConfig config_of(char* prog, char* inp, int model)
 {
     InputParser ip;
-    add_options(ip, 0);
-    char* args[] = {argv[0], (char*)"-i", argv[2]};
-    if (!ip.Parse(3, args, false)) return 3;
-    RunnerConfig c = read_config(ip, 0);
+    add_options(ip, model);
+    char* args[] = {prog, (char*)"-i", inp};
+    if (!ip.Parse(3, args, false)) exit(3);
+    return read_config(ip, model);
+}
+// import <inp> <restart base> <rank> <W> <image out> [model [event capacity]]: the
+// configuration of the input file, the restart file of (rank, trial 0) as a
+// checkpoint image
+static int import_mode(int argc, char** argv)
+{
+    const int model = argc > 7 ? atoi(argv[7]) : 0;
+    RunnerConfig c = config_of(argv[0], argv[2], model);
     c.input_restart = argv[3];
     ImageSpec spec;
     spec.seed = 77;
@@ -42,6 +49,10 @@ static int import_mode(char** argv)
     spec.barrier_window = 4096;
     spec.n_fields = c.write_x ? 7 : 1;
     spec.tick_capacity = 1 << 16;
+    if (argc > 8) {
+        spec.event_capacity = strtoull(argv[8], nullptr, 10);
+        spec.n_lists = LDPCP_LIST_CG0 + (c.fptcg ? (uint32_t)c.cg_size.size() : 0u);
+    }
     std::vector<char> image;
     uint64_t n = 0;
     if (!read_restart_file(c, atoi(argv[5]), atoi(argv[4]), 0, spec, image, &n)) return 2;
@@ -51,9 +62,39 @@ static int import_mode(char** argv)
     printf("%llu\n", (unsigned long long)n);
     return 0;
 }
+// export <inp> <model> <image> <tables> <restart base> <rank> <W>: the image
+// written in the reference's layout (trial 0) with the barrier tables of the
+// file <tables> (per array: int32 lo, int32 n, states [W][n] int32, steps
+// [W][n] uint64)
+static int export_mode(char** argv)
+{
+    const int model = atoi(argv[3]);
+    RunnerConfig c = config_of(argv[0], argv[2], model);
+    c.output_restart = argv[6];
+    const int W = atoi(argv[8]);
+    std::vector<char> image;
+    FILE* f = fopen(argv[4], "rb");
+    for (int ch; (ch = fgetc(f)) != EOF;) image.push_back((char)ch);
+    fclose(f);
+    if (!reference_layout_possible(c, image)) return 4;
+    std::vector<BarrierTable> tables(model ? 2 : 1);
+    f = fopen(argv[5], "rb");
+    for (BarrierTable& T : tables) {
+        if (fread(&T.lo, 4, 1, f) != 1 || fread(&T.n, 4, 1, f) != 1) return 5;
+        T.states.resize((size_t)W * T.n);
+        T.steps.resize((size_t)W * T.n);
+        if (fread(T.states.data(), 4, T.states.size(), f) != T.states.size()) return 5;
+        if (fread(T.steps.data(), 8, T.steps.size(), f) != T.steps.size()) return 5;
+    }
+    fclose(f);
+    ldpcp_checkpoint_header hd;
+    memcpy(&hd, image.data(), sizeof(hd));
+    return write_reference_restart_file(c, W, atoi(argv[7]), 0, hd.steps_done, image, tables) ? 0 : 1;
+}
 int main(int argc, char** argv)
 {
-    if (!strcmp(argv[1], "import")) return import_mode(argv);
+    if (!strcmp(argv[1], "import")) return import_mode(argc, argv);
+    if (!strcmp(argv[1], "export")) return export_mode(argv);
     RunnerConfig c;
     c.model = atoi(argv[2]);
     c.x0 = -1.05; c.v0 = 0.5; c.gamma = 2.0; c.kb = 1.5; c.temperature = 0.75; c.mass = 3.0;
@@ -172,8 +213,18 @@ def parse_image(buf):
     frames = np.frombuffer(buf, np.float64, hd.n_fields * nf * n, o).reshape(hd.n_fields, nf, n)
     o += hd.n_fields * nf * n * 8
     ticks = np.frombuffer(buf, np.float64, n * cap, o).reshape(n, cap)
-    assert o + n * cap * 8 == len(buf) == hd.bytes
+    o += n * cap * 8
+    if hd.n_lists:   # the event lists, trajectory after trajectory, list after list
+        counts = [min(_list_count(lanes[t], k), hd.event_capacity) for t in range(n) for k in range(hd.n_lists)]
+        o += 8 * sum(counts)
+    assert o == len(buf) == hd.bytes
     return hd, lanes, wbase, win, frames, ticks
+
+
+def _list_count(lane, k):
+    """ldpcp_list_count: FPT1 and its two barrier logs share fpt_n."""
+    return {0: lane.fpt_n, 1: lane.fpt2_n, 2: lane.tpt_n, 3: lane.tpt2_n, 4: lane.fpt_n,
+            5: lane.fpt_n}.get(k, lane.cg_n[k - 6] if k >= 6 else 0)
 
 
 @pytest.mark.skipif(not os.path.isdir(QUICK_JOB), reason="the reference's recorded quick-test job is not on this machine")
@@ -257,6 +308,212 @@ def test_host_restart_code_is_clean_under_sanitizers(tmp_path):
     path = tmp_path / "rs_00001_00002.restart"
     path.write_bytes(path.read_bytes()[:-10])
     go("read", "0", rc=2)
+    # the double barrier's reference layout: import, export, and the exported file read back
+    import numpy as np
+    import restart_decode as R
+    rng = np.random.default_rng(4)
+    W, n = 4, 301
+    xc = -1.0
+    lanes = [_double_lane(rng, i, 3 * 10**9, 11, [2, 3, 4, 5, 10, 100, 1000], xc) for i in range(W)]
+    tab = [(rng.integers(0, 2, (n, W)).astype(np.int32), rng.integers(0, 10**6, (n, W)).astype(np.uint64))
+           for _ in range(2)]
+    h = {"signature": 2345, "simd_width": W, "x0": -2.9, "v0": 1.0, "gamma": 1.0, "kb": 1.0, "temperature": 1.0,
+         "mass": 1.0, "timestep": 1e-4, "nsteps_total": 3 * 10**9, "waiting_time": 1, "stat_freq": 1,
+         "save_freq": 300000, "save_flags": [1, 0, 0, 0], "randomize_v": 1, "randomize_eta": 1,
+         "periodic_length": 5.8, "h00": 3.4, "h01": 2.0, "h10": 7.2, "h11": 2.0, "initial_state": 0.0,
+         "position0": -1.42, "position1": -0.56, "xc": xc, "randomize_state": 1, "same_initial_barrier": 0,
+         "q": 0.2, "tau": 1e-5, "eta0": 0.0, "save_flags2": [0, 0, 0], "rank": 0, "rng": [(0, 0)] * 3}
+    R.encode({"header": h, "lanes": lanes, "barrier_state": tab[0][0], "barrier_step": tab[0][1],
+              "barrier_min_id": -150, "barrier_state1": tab[1][0], "barrier_step1": tab[1][1],
+              "barrier_min_id1": -150, "r_prev": rng.normal(size=W)}, str(tmp_path / "dref_00000_00000.restart"),
+             model=1)
+    inp = os.path.join(GOLDEN, "cuidb_double_2.0.inp")
+    go("import", inp, "dref", "0", str(W), str(tmp_path / "dimg"), "1", "65536")
+    with open(tmp_path / "dtab", "wb") as f:
+        for st, sp in tab:
+            f.write(np.array([-150, n], np.int32).tobytes())
+            f.write(np.ascontiguousarray(st.T).tobytes())
+            f.write(np.ascontiguousarray(sp.T).tobytes())
+    go("export", inp, "1", str(tmp_path / "dimg"), str(tmp_path / "dtab"), "dout", "0", str(W))
+    go("import", inp, "dout", "0", str(W), str(tmp_path / "dimg2"), "1", "65536")
+    assert (tmp_path / "dimg2").read_bytes() == (tmp_path / "dimg").read_bytes()
     if os.path.isdir(QUICK_JOB):
         go("import", os.path.join(GOLDEN, "quick_test.inp"), os.path.join(QUICK_JOB, "m1.0_t1.00e-05_restart_0"),
            "2", "4", str(tmp_path / "img"))
+
+
+def _double_lane(rng, W_i, nsteps, nf, cg_sizes, xc):
+    """One lane of a double-barrier restart record in the conventions the
+    writer follows (previous values equal current ones, flags clear, eta 0)."""
+    import numpy as np
+    x = float(rng.uniform(-9.0, 9.0))
+    periodic = float(np.rint(x * (1.0 / 5.8)))
+    well = float(np.floor(x * (1.0 / 5.8)))
+    xp = x - periodic * 5.8
+    local = 0.0 if xp < xc else 1.0
+    state = float(rng.integers(0, 2))
+    c0 = int(rng.integers(0, nsteps))
+    nfpt = int(rng.integers(0, 40))
+    L = {"x": x, "v": 0.0, "f": float(rng.normal()), "e": float(rng.uniform(0, 7)), "eta": 0.0,
+         "barrier": float(rng.uniform(0, 7)), "xp": xp,
+         "traj_x": rng.normal(size=nf), "current_state": state, "previous_state": state,
+         "slope_pos0": 0.0, "slope_pos1": 0.0, "slope_neg0": 0.0, "slope_neg1": 0.0,
+         "intercept_pos0": 0.0, "intercept_pos1": 0.0, "intercept_neg0": 0.0, "intercept_neg1": 0.0,
+         "current_height0": 0.0, "current_height1": 0.0,
+         "state_count0": float(c0), "state_count1": float(nsteps + 1 - c0),
+         "well": well, "periodic": periodic, "well_prev": well, "periodic_prev": periodic,
+         "entering_well": 0, "entering_period": 0,
+         "local": local, "local_prev": local, "entering_local": 0, "local_stored": local,
+         "local_prev_stored": local, "entering_local_stored": 0,
+         "v0_instance": float(rng.normal()), "eta0_instance": 0.0, "state0_instance": float(rng.integers(0, 2)),
+         "well_stored": well, "periodic_stored": periodic, "well_prev_stored": well, "periodic_prev_stored": periodic,
+         "entering_well_stored": 0, "entering_period_stored": 0,
+         "tick_prev": float(rng.integers(0, nsteps)), "barrier_passed": int(rng.integers(0, 2)),
+         "barrier_ticks": np.sort(rng.integers(0, nsteps, int(rng.integers(0, 50)))).astype(np.float64),
+         "fpt": rng.exponential(3.0, nfpt), "instant_barrier": rng.integers(0, 2, nfpt).astype(np.float64),
+         "initial_barrier": rng.integers(0, 2, nfpt).astype(np.float64),
+         "fpt_last_tick": float(rng.uniform(0, 3e5)), "fpt_to_update": int(rng.integers(0, 2)),
+         "fpt_init_state": (float(rng.integers(0, 2)), float(rng.integers(0, 2))),
+         "fpt_init_index": (int(rng.integers(-5, 5)), int(rng.integers(-5, 5))),
+         "fpt2": rng.exponential(30.0, int(rng.integers(0, 60))), "fpt2_last_tick": float(rng.uniform(0, 3e5)),
+         "fpt2_to_update": int(rng.integers(0, 2)), "fpt2_index": (int(rng.integers(-5, 5)), int(rng.integers(-5, 5))),
+         "tpt": rng.exponential(0.5, int(rng.integers(0, 60))), "tpt_last_tick": float(rng.uniform(0, 3e5)),
+         "tpt_to_update": int(rng.integers(0, 2)),
+         "tpt2": rng.exponential(0.5, int(rng.integers(0, 60))), "tpt2_last_tick": float(rng.uniform(0, 3e5)),
+         "tpt2_to_update": int(rng.integers(0, 2)), "tpt2_initial_well": int(well),
+         "cg_sizes": np.array(cg_sizes, np.int32), "cg": []}
+    for size in cg_sizes:
+        L["cg"].append({"list": rng.exponential(40.0 * size, int(rng.integers(0, 30))),
+                        "last_tick": float(rng.uniform(0, 3e5)), "to_update": int(rng.integers(0, 2)),
+                        "initial_well": int(rng.integers(-3, 3))})
+    for k in ("traj_v", "traj_f", "traj_pe", "traj_ke", "traj_eta", "traj_state", "traj_barrier"):
+        L[k] = np.zeros(0)
+    return L
+
+
+def test_double_barrier_reference_layout_round_trip(harness, tmp_path):
+    """The double-barrier program's own restart layout
+    (langevinDynamicsDoubleBarrier.h:414-642 and 1202-1294; two barrier tables,
+    the marker 9948, R_prev): a file in that layout (written by the test's
+    encoder) imports into a checkpoint image holding every lane's state,
+    local-barrier index, running sums and both barrier arrays; the image
+    written back by the program's writer decodes to the same record, lane by
+    lane, with the potential constants the reference derives
+    (Double.h:1845-1899) continuous at every segment end; that file, read
+    back, gives the identical image (taken from after its end marker), and a
+    damaged table marker is refused."""
+    import numpy as np
+    import restart_decode as R
+    rng = np.random.default_rng(3)
+    W, rank, nsteps, nf = 4, 1, 3 * 10**9, 11
+    cg_sizes = [2, 3, 4, 5, 10, 100, 1000]
+    x1b, w1, x2b, w2, y1, y2 = -1.42, 0.70, -0.56, 0.65, 3.40, 7.20
+    x1h, x2l = x1b + w1, x2b - w2
+    s1, s2 = -y1 / (x1h - x1b), y2 / (x2b - x2l)          # falling and rising inner slopes (state 0)
+    xc_exact = (x1h * s1 - x2l * s2) / (s1 - s2)
+    header = {"signature": 2345, "simd_width": W, "x0": -2.9, "v0": 1.0, "gamma": 1.0, "kb": 1.0,
+              "temperature": 1.0, "mass": 1.0, "timestep": 1e-4, "nsteps_total": nsteps, "waiting_time": 1,
+              "stat_freq": 1, "save_freq": 300000, "save_flags": [1, 0, 0, 0], "randomize_v": 1, "randomize_eta": 1,
+              "periodic_length": 5.8, "h00": 3.4, "h01": 2.0, "h10": 7.2, "h11": 2.0, "initial_state": 0.0,
+              "position0": x1b, "position1": x2b, "xc": xc_exact, "randomize_state": 1, "same_initial_barrier": 0,
+              "q": 0.2, "tau": 1e-5, "eta0": 0.0, "save_flags2": [0, 0, 0], "rank": rank,
+              "rng": [(nsteps, 0), (nsteps, 0), (0, 0)]}
+    lanes = [_double_lane(rng, i, nsteps, nf, cg_sizes, xc_exact) for i in range(W)]
+    tables = []
+    for a in range(2):
+        n = 20001
+        tables.append((rng.integers(0, 2, (n, W)).astype(np.int32), rng.integers(0, nsteps, (n, W)).astype(np.uint64)))
+    d = {"header": header, "lanes": lanes, "barrier_state": tables[0][0], "barrier_step": tables[0][1],
+         "barrier_min_id": -10000, "barrier_state1": tables[1][0], "barrier_step1": tables[1][1],
+         "barrier_min_id1": -10000, "r_prev": rng.normal(size=W)}
+    R.encode(d, str(tmp_path / f"ref_{rank:05d}_00000.restart"), model=1)
+    assert R.decode(str(tmp_path / f"ref_{rank:05d}_00000.restart"), model=1)["trailer"] is None
+    inp = os.path.join(GOLDEN, "cuidb_double_2.0.inp")
+
+    # import: the reference's record -> checkpoint image
+    r = subprocess.run([str(harness), "import", inp, "ref", str(rank), str(W), str(tmp_path / "img"), "1", "65536"],
+                       cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert int(r.stdout) == nsteps
+    image = (tmp_path / "img").read_bytes()
+    hd, got, wbase, win, frames, ticks = parse_image(image)
+    assert (hd.model, hd.overdamped, hd.steps_done, hd.n_frames, hd.events_complete) == (1, 1, nsteps, nf, 1)
+    for i, L in enumerate(lanes):
+        G = got[i]
+        assert (G.x, G.xp, G.local_barrier, G.state, G.well, G.periodic) == \
+            (L["x"], L["xp"], L["local"], L["current_state"], L["well"], L["periodic"])
+        assert (G.fpt2_n, G.tpt_n, G.tpt2_n, G.wt_ticks) == (len(L["fpt2"]), len(L["tpt"]), len(L["tpt2"]),
+                                                             len(L["barrier_ticks"]))
+        assert G.r_prev == d["r_prev"][i]
+        np.testing.assert_array_equal(frames[0, :, i], L["traj_x"])
+        for a in range(2):
+            for k in range(20001):
+                slot = -10000 + k - wbase[i]
+                if 0 <= slot < hd.barrier_window:
+                    e = int(win[i, a, slot])
+                    assert e >> 63 == 1 and (e >> 62) & 1 == tables[a][0][k, i]
+                    assert e & ((1 << 62) - 1) == tables[a][1][k, i]
+
+    # export: the image in the reference's layout, with the same tables
+    with open(tmp_path / "tables", "wb") as f:
+        for st, sp in tables:
+            f.write(np.array([-10000, st.shape[0]], np.int32).tobytes())
+            f.write(np.ascontiguousarray(st.T).tobytes())
+            f.write(np.ascontiguousarray(sp.T).tobytes())
+    r = subprocess.run([str(harness), "export", inp, "1", str(tmp_path / "img"), str(tmp_path / "tables"), "out",
+                        str(rank), str(W)], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    e = R.decode(str(tmp_path / f"out_{rank:05d}_00000.restart"), model=1)
+    assert e["trailer"] == image
+    assert abs(e["header"]["xc"] - xc_exact) < 1e-12
+    assert {k: v for k, v in e["header"].items() if k != "xc"} == {k: v for k, v in header.items() if k != "xc"}
+    xc = e["header"]["xc"]
+    yc = (x1h - xc) / (x1h - x1b) * y1
+    derived = {k for k in lanes[0] if k.startswith(("slope_", "intercept_", "current_height"))}
+    for L, M in zip(lanes, e["lanes"]):
+        for k, v in L.items():
+            if k in derived:
+                continue
+            if k == "cg":
+                for a, b in zip(v, M[k]):
+                    assert np.array_equal(a["list"], b["list"])
+                    assert (a["last_tick"], a["to_update"], a["initial_well"]) == \
+                        (b["last_tick"], b["to_update"], b["initial_well"])
+            elif isinstance(v, np.ndarray):
+                np.testing.assert_array_equal(M[k], v, err_msg=k)
+            else:
+                assert tuple(np.atleast_1d(M[k])) == tuple(np.atleast_1d(v)), k
+        # E = xp * slope + intercept on each segment (Double.h:1498-1513): zero at the
+        # outer feet, the barrier heights at the barrier tops, yc at the apex xc
+        s = int(L["current_state"])
+        h0, h1 = (3.4, 2.0)[s], (7.2, 2.0)[s]
+        assert (M["current_height0"], M["current_height1"]) == (h0, h1)
+        for slope, icpt, x, want in (("slope_neg0", "intercept_neg0", x1b - w1, 0.0),
+                                     ("slope_neg0", "intercept_neg0", x1b, h0),
+                                     ("slope_pos0", "intercept_pos0", x1b, h0),
+                                     ("slope_pos0", "intercept_pos0", xc, yc),
+                                     ("slope_neg1", "intercept_neg1", xc, yc),
+                                     ("slope_neg1", "intercept_neg1", x2b, h1),
+                                     ("slope_pos1", "intercept_pos1", x2b, h1),
+                                     ("slope_pos1", "intercept_pos1", x2b + w2, 0.0)):
+            term = abs(x * M[slope]) + abs(M[icpt])
+            assert abs(x * M[slope] + M[icpt] - want) <= 8 * np.finfo(float).eps * term, (slope, x, s)
+    for key in ("barrier_state", "barrier_step", "barrier_state1", "barrier_step1", "r_prev"):
+        np.testing.assert_array_equal(e[key], d[key], err_msg=key)
+    assert (e["barrier_min_id"], e["barrier_min_id1"]) == (-10000, -10000)
+
+    # this program's reader takes the exact image after the end marker
+    r = subprocess.run([str(harness), "import", inp, "out", str(rank), str(W), str(tmp_path / "img2"), "1", "65536"],
+                       cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "img2").read_bytes() == image
+
+    # a damaged marker after the two tables is refused
+    buf = bytearray((tmp_path / f"ref_{rank:05d}_00000.restart").read_bytes())
+    pos = len(buf) - 4 - 8 * W - 4
+    assert struct.unpack_from("<i", buf, pos)[0] == 9948
+    buf[pos:pos + 4] = struct.pack("<i", 9949)
+    (tmp_path / f"ref_{rank:05d}_00000.restart").write_bytes(bytes(buf))
+    r = subprocess.run([str(harness), "import", inp, "ref", str(rank), str(W), str(tmp_path / "img3"), "1", "65536"],
+                       cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 2 and "Error: wrong signiture" in r.stderr
