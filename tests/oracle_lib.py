@@ -178,3 +178,62 @@ class MklStreams:
 
 def mkl_available():
     return os.path.exists(MKL_RT)
+
+
+SIMD_SO = {8: os.path.join(ORACLE_DIR, "_build", "libldpcp_simd512.so"),
+           4: os.path.join(ORACLE_DIR, "_build", "libldpcp_simd256.so")}
+
+
+def host_simd_width():
+    """8 where the host has AVX-512F and -DQ, 4 where it has AVX2 and FMA, else 0."""
+    try:
+        flags = set()
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+    except OSError:
+        return 0
+    if {"avx512f", "avx512dq"} <= flags:
+        return 8
+    if {"avx2", "fma"} <= flags:
+        return 4
+    return 0
+
+
+def load_simd(width=None):
+    """The SIMD CPU baseline (oracle/cpu_simd.c) built for `width` lanes
+    (default: the widest the host runs)."""
+    width = width or host_simd_width()
+    if width not in SIMD_SO:
+        raise RuntimeError("the host has neither AVX-512 nor AVX2+FMA")
+    if not os.path.exists(SIMD_SO[width]):
+        build_oracle()
+    lib = C.CDLL(SIMD_SO[width])
+    U64P = C.POINTER(C.c_uint64)
+    lib.ldo_simd_width.restype = C.c_int
+    lib.ldo_simd_absorb.restype = C.c_int
+    lib.ldo_simd_absorb.argtypes = [C.POINTER(Params), U64P, C.c_int, C.c_uint32, C.c_double, C.c_int,
+                                    U64P, U64P, U64P, U64P, C.POINTER(C.c_double)]
+    return lib
+
+
+def simd_absorb(lib, params, seeds, n_traj=0, seconds=0.0, threads=1, want_paths=True):
+    """Run ldo_simd_absorb over parameter sets `params` (list of Params) with
+    Philox seeds `seeds`; returns (absorbed[set][traj], tpt0[...], traj_steps,
+    traj_done, wall)."""
+    n = len(params)
+    P = (Params * n)(*params)
+    S = (C.c_uint64 * n)(*seeds)
+    ab = np.zeros((n, max(n_traj, 1)), dtype=np.uint64)
+    tp = np.zeros_like(ab)
+    steps, done, wall = C.c_uint64(), C.c_uint64(), C.c_double()
+    u64p = C.POINTER(C.c_uint64)
+    rc = lib.ldo_simd_absorb(P, S, n, n_traj, seconds, threads,
+                             ab.ctypes.data_as(u64p) if want_paths and n_traj else None,
+                             tp.ctypes.data_as(u64p) if want_paths and n_traj else None,
+                             C.byref(steps), C.byref(done), C.byref(wall))
+    if rc != 0:
+        raise RuntimeError(f"ldo_simd_absorb returned {rc}")
+    return ab, tp, steps.value, done.value, wall.value
