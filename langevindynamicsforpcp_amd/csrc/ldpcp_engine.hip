@@ -59,7 +59,7 @@ struct SetDev {
     double x0;
     float fL, fhalfL, fw, fS[2], fdtgm, fvar, fminv_halfdt, fhalfdt, fexpmgdt, fc_lap, fK;
     float fxb[2], flo[2], fhi[2], fxc, fslope[2][2][2];
-    float fxp0, fpad;
+    float fxp0, fbm;                                // fbm = -2 ln2 var^2 (scaled FP32 Box-Muller)
     uint64_t T_flip, T_asym[2], T_create, T_p0le;  // integer thresholds on Philox words
     uint32_t Tl[2];                                 // per-step flip thresholds leaving state 0 / 1, below 2^32
     int32_t Tall[2];                                // 1: that threshold is 2^32 (always flip)
@@ -103,6 +103,28 @@ __device__ __forceinline__ void box_muller_f32(uint32_t a, uint32_t b, float& g0
     g1 = r * __builtin_amdgcn_sinf(u2);
 }
 
+// The step loop's pair of Gaussians already multiplied by the noise amplitude
+// s: r = sqrt(c log2 u1) with c = -2 ln2 s^2 (one multiply by a scalar), and
+// u = w 2^-32 + 2^-33 in one fused multiply-add.
+__device__ __forceinline__ void box_muller_f32_scaled(uint32_t a, uint32_t b, float c, float& g0, float& g1)
+{
+    const float u1 = __builtin_fmaf(__uint2float_rn(a), 2.3283064365386963e-10f, 1.16415321826934814e-10f);
+    const float u2 = __builtin_fmaf(__uint2float_rn(b), 2.3283064365386963e-10f, 1.16415321826934814e-10f);
+    const float r = __builtin_amdgcn_sqrtf(c * __builtin_amdgcn_logf(u1));
+    g0 = r * __builtin_amdgcn_cosf(u2);
+    g1 = r * __builtin_amdgcn_sinf(u2);
+}
+
+// A wave-uniform value copied to a vector register: a per-lane select between
+// it and a scalar is then one v_cndmask (a select reads at most one scalar
+// operand; two scalars cost two extra moves on every use).
+__device__ __forceinline__ float in_vgpr(float x)
+{
+    float v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+    return v;
+}
+
 __device__ __forceinline__ bool below(uint32_t w, uint64_t T) { return (uint64_t)w < T; }
 
 // Wave-uniform copy of a set: the fields the step loop touches every step are
@@ -133,7 +155,7 @@ struct Hot {
     uint32_t Tl0, Tl1;    // per-step flip thresholds for leaving state 0 / 1 ...
     int32_t all0, all1;   // ... or 1 when the threshold is 2^32 (always flip)
     // FP32 engine
-    float fL, fhalfL, fdtgm, fminv_halfdt, fhalfdt, fexpmgdt, fvar, fw, fS0, fS1, fxc;
+    float fL, fhalfL, fdtgm, fminv_halfdt, fhalfdt, fexpmgdt, fvar, fbm, fw, fS0, fS1, fxc;
     float fxb[2], flo[2], fhi[2], fsl[2][2][2];  // fsl[barrier][state][side]
     // FP64 engine
     double L, Linv, dtgm, minv, halfdt, expmgdt, var, w, S0, S1, xc;
@@ -167,6 +189,7 @@ __device__ __forceinline__ Hot load_hot(const SetDev* __restrict__ g)
             h.fexpmgdt = uni(g->fexpmgdt);
         }
         h.fvar = uni(g->fvar);
+        h.fbm = uni(g->fbm);
         if (MODEL == 0) {
             h.fw = uni(g->fw);
             h.fS0 = uni(g->fS[0]);
@@ -363,19 +386,39 @@ struct Window {
 // ---------------------------------------------------------------------------
 // FP32 stepping.  MODEL 0 single, 1 double barrier.
 // ---------------------------------------------------------------------------
+// state-0 slopes held in vector registers (in_vgpr), so that choosing the
+// slope of the lane's state is one select
+struct Slope0 {
+    float s0;        // single barrier
+    float a[2][2];   // double barrier [barrier][side]
+};
+
 template <int MODEL>
-__device__ __forceinline__ float force_f32(const Hot& H, float xp, int state)
+__device__ __forceinline__ Slope0 slope0_f32(const Hot& H)
+{
+    Slope0 z;
+    if (MODEL == 0) {
+        z.s0 = in_vgpr(H.fS0);
+    } else {
+        for (int b = 0; b < 2; ++b)
+            for (int side = 0; side < 2; ++side) z.a[b][side] = in_vgpr(H.fsl[b][0][side]);
+    }
+    return z;
+}
+
+template <int MODEL>
+__device__ __forceinline__ float force_f32(const Hot& H, const Slope0& z, float xp, int state)
 {
     if (MODEL == 0) {
         // -slope on (-w, 0), +slope on (0, w), 0 elsewhere (strict compares,
         // langevinDynamicsSingleBarrier.h:1292-1324): the slope with xp's sign bit
         // xor-ed in, where 0 < |xp| < w
-        const float sp = state ? H.fS1 : H.fS0;
+        const float sp = state ? H.fS1 : z.s0;
         const float signed_sp = __uint_as_float(__float_as_uint(sp) ^ (__float_as_uint(xp) & 0x80000000u));
         return (fabsf(xp) < H.fw && xp != 0.0f) ? signed_sp : 0.0f;
     } else {
-        const float a0 = state ? H.fsl[0][1][0] : H.fsl[0][0][0], a1 = state ? H.fsl[0][1][1] : H.fsl[0][0][1];
-        const float b0 = state ? H.fsl[1][1][0] : H.fsl[1][0][0], b1 = state ? H.fsl[1][1][1] : H.fsl[1][0][1];
+        const float a0 = state ? H.fsl[0][1][0] : z.a[0][0], a1 = state ? H.fsl[0][1][1] : z.a[0][1];
+        const float b0 = state ? H.fsl[1][1][0] : z.a[1][0], b1 = state ? H.fsl[1][1][1] : z.a[1][1];
         float s = 0.0f;
         s = (xp < H.fxb[0] && xp > H.flo[0]) ? a0 : s;
         s = (xp < H.fxc && xp > H.fxb[0]) ? a1 : s;
@@ -408,52 +451,34 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
     }
     int state = flags & F_STATE;
     int local = (MODEL == 1) ? (xp < H.fxc ? 0 : 1) : 0;
-    bool neg = xp < 0.0f;
-    float F = force_f32<MODEL>(H, xp, state);
+    const Slope0 z = slope0_f32<MODEL>(H);
+    float F = force_f32<MODEL>(H, z, xp, state);
     int j = W.j0;
     uint32_t next_lag = j < LDPCP_MSD_LAGS ? Sg->lag[j] : kNoLag;
-    float g_odd = 0.0f;
-    uint32_t w_odd = 0;
     // the step counter is wave-uniform; the exit step of a lane is kept apart
     // so that the counter is not live after the loop (it stays scalar)
     uint32_t last = W.s_end;
     bool done = false;
-    for (uint32_t step = W.s0 + 1; step <= W.s_end; ++step) {
-        float g;
-        uint32_t wflip;
-        if ((step & 1u) == 1u) {
-            const uint32_t q = (step - 1) >> 1;
-            const Philox4 w = philox4x32_10(q, 0u, tid, 0u, H.key0, H.key1);
-            box_muller_f32(w.w[0], w.w[1], g, g_odd);
-            wflip = w.w[2];
-            w_odd = w.w[3];
-        } else {
-            if (step == W.s0 + 1) {
-                const uint32_t q = (step - 1) >> 1;
-                const Philox4 w = philox4x32_10(q, 0u, tid, 0u, H.key0, H.key1);
-                float g0;
-                box_muller_f32(w.w[0], w.w[1], g0, g_odd);
-                w_odd = w.w[3];
-            }
-            g = g_odd;
-            wflip = w_odd;
-        }
+    // one step with Gaussian g and flip word wflip; true when the lane is absorbed
+    auto one = [&](const uint32_t step, const float g, const uint32_t wflip) __attribute__((always_inline)) {
         // position increment (overdamped MA(1) scheme, langevinDynamicsSingleBarrier.h:2860-2876;
         // underdamped B-A-O-A-B, :1680-1706), compensated offset update
+        // (g carries the noise amplitude: box_muller_f32_scaled)
         float dx;
         if (OVER) {
-            const float rf = H.fvar * g;
-            dx = __builtin_fmaf(F, H.fdtgm, rf + R);
-            R = rf;
+            dx = __builtin_fmaf(F, H.fdtgm, g + R);
+            R = g;
         } else {
             v = __builtin_fmaf(F, H.fminv_halfdt, v);
             const float d1 = v * H.fhalfdt;
-            v = __builtin_fmaf(H.fexpmgdt, v, H.fvar * g);
+            v = __builtin_fmaf(H.fexpmgdt, v, g);
             dx = __builtin_fmaf(v, H.fhalfdt, d1);
         }
         const float y = dx - comp;
         const float t = xp + y;
         comp = (t - xp) - y;
+        // the offset's sign changes within a cell: a barrier-top crossing
+        const bool sign_change = (int32_t)(__float_as_uint(t) ^ __float_as_uint(xp)) < 0;
         xp = t;
         // periodic (well-centre) crossing: offset kept in [-L/2, L/2)
         bool ent_period = false;
@@ -464,10 +489,8 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
             cell = up ? cell + 1 : cell - 1;
             ent_period = true;
         }
-        // well (barrier-top) crossing: the sign of the offset changes within a cell
-        const bool neg_before = neg;
-        neg = xp < 0.0f;
-        const bool ent_well = !ent_period && (neg_before != neg);
+        // well (barrier-top) crossing
+        const bool ent_well = !ent_period && sign_change;
         state = flip_step<EQ>(H, state, wflip);
         if (MODEL == 0) {
             if (ent_period) {
@@ -488,28 +511,67 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
                 }
             }
         }
-        F = force_f32<MODEL>(H, xp, state);
+        F = force_f32<MODEL>(H, z, xp, state);
         if (!OVER) v = __builtin_fmaf(F, H.fminv_halfdt, v);
         if (step == next_lag) {
             const double x = (double)cell * (double)H.fL + (double)xp - (double)comp;
             const double d = x - Sg->x0;
             atomicAdd(lds_sum + j, d * d);
             atomicAdd(lds_n + j, 1u);
-            ++j;
-            next_lag = j < LDPCP_MSD_LAGS ? Sg->lag[j] : kNoLag;
+            // every lane still stepping has passed the same lags: j and the next lag
+            // stay wave-uniform (scalar) inside the loop
+            j = uni(j + 1);
+            next_lag = uni(j < LDPCP_MSD_LAGS ? Sg->lag[j] : kNoLag);
         }
         // single barrier: first barrier-top crossing; double: first well centre
         // outside the two cells of the start
         const bool hit =
             (MODEL == 0) ? ent_well : (ent_period && (unsigned)(cell - H.slot_base) > 1u);
-        if (hit) {
-            done = true;
-            last = step;
-            break;
-        }
-        if (ent_period) {
+        if (!hit && ent_period) {
             tpt0 = step;
             if (MODEL == 1) xf = 0;
+        }
+        return hit;
+    };
+    // Steps 2q+1 and 2q+2 share one Philox call (role 0, counter q): the loop
+    // runs over whole pairs, with half a pair before it when the window starts
+    // after an odd step and after it when the window ends on one.
+    uint32_t begin = W.s0;
+    if ((begin & 1u) && begin < W.s_end) {
+        const Philox4 w = philox4x32_10(begin >> 1, 0u, tid, 0u, H.key0, H.key1);
+        float g0, g1;
+        box_muller_f32_scaled(w.w[0], w.w[1], H.fbm, g0, g1);
+        ++begin;
+        if (one(begin, g1, w.w[3])) {
+            done = true;
+            last = begin;
+        }
+    }
+    const uint32_t pair_end = begin + ((W.s_end - begin) & ~1u);
+    if (!done) {
+        for (uint32_t step = begin; step < pair_end; step += 2) {
+            const Philox4 w = philox4x32_10(step >> 1, 0u, tid, 0u, H.key0, H.key1);
+            float g0, g1;
+            box_muller_f32_scaled(w.w[0], w.w[1], H.fbm, g0, g1);
+            if (one(step + 1, g0, w.w[2])) {
+                done = true;
+                last = step + 1;
+                break;
+            }
+            if (one(step + 2, g1, w.w[3])) {
+                done = true;
+                last = step + 2;
+                break;
+            }
+        }
+    }
+    if (!done && pair_end < W.s_end) {
+        const Philox4 w = philox4x32_10(pair_end >> 1, 0u, tid, 0u, H.key0, H.key1);
+        float g0, g1;
+        box_muller_f32_scaled(w.w[0], w.w[1], H.fbm, g0, g1);
+        if (one(pair_end + 1, g0, w.w[2])) {
+            done = true;
+            last = pair_end + 1;
         }
     }
     if (done) P.absorbed[slot] = last;
@@ -586,30 +648,10 @@ __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St,
     double F = force_f64<MODEL>(H, x, periodic, state);
     int j = W.j0;
     uint32_t next_lag = j < LDPCP_MSD_LAGS ? Sg->lag[j] : kNoLag;
-    double g_odd = 0.0;
-    uint32_t w_odd = 0;
     uint32_t last = W.s_end;
     bool done = false;
-    for (uint32_t step = W.s0 + 1; step <= W.s_end; ++step) {
-        double g;
-        uint32_t wflip;
-        if ((step & 1u) == 1u) {
-            const uint32_t q = (step - 1) >> 1;
-            const Philox4 w = philox4x32_10(q, 0u, tid, 0u, H.key0, H.key1);
-            box_muller_f64(w.w[0], w.w[1], g, g_odd);
-            wflip = w.w[2];
-            w_odd = w.w[3];
-        } else {
-            if (step == W.s0 + 1) {
-                const uint32_t q = (step - 1) >> 1;
-                const Philox4 w = philox4x32_10(q, 0u, tid, 0u, H.key0, H.key1);
-                double g0;
-                box_muller_f64(w.w[0], w.w[1], g0, g_odd);
-                w_odd = w.w[3];
-            }
-            g = g_odd;
-            wflip = w_odd;
-        }
+    // one step with Gaussian g and flip word wflip; true when the lane is absorbed
+    auto one = [&](const uint32_t step, const double g, const uint32_t wflip) __attribute__((always_inline)) {
         if (OVER) {
             const double rf = H.var * g;
             x = fma(F, H.dtgm, x);
@@ -654,19 +696,56 @@ __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St,
             const double d = x - Sg->x0;
             atomicAdd(lds_sum + j, d * d);
             atomicAdd(lds_n + j, 1u);
-            ++j;
-            next_lag = j < LDPCP_MSD_LAGS ? Sg->lag[j] : kNoLag;
+            // every lane still stepping has passed the same lags: j and the next lag
+            // stay wave-uniform (scalar) inside the loop
+            j = uni(j + 1);
+            next_lag = uni(j < LDPCP_MSD_LAGS ? Sg->lag[j] : kNoLag);
         }
         const int kcell = (int)periodic - H.slot_base;
         const bool hit = (MODEL == 0) ? ent_well : (ent_period && (kcell < 0 || kcell > 1));
-        if (hit) {
-            done = true;
-            last = step;
-            break;
-        }
-        if (ent_period) {
+        if (!hit && ent_period) {
             tpt0 = step;
             if (MODEL == 1) xf = 0;
+        }
+        return hit;
+    };
+    // step pairs sharing one Philox call, as in body_f32
+    uint32_t begin = W.s0;
+    if ((begin & 1u) && begin < W.s_end) {
+        const Philox4 w = philox4x32_10(begin >> 1, 0u, tid, 0u, H.key0, H.key1);
+        double g0, g1;
+        box_muller_f64(w.w[0], w.w[1], g0, g1);
+        ++begin;
+        if (one(begin, g1, w.w[3])) {
+            done = true;
+            last = begin;
+        }
+    }
+    const uint32_t pair_end = begin + ((W.s_end - begin) & ~1u);
+    if (!done) {
+        for (uint32_t step = begin; step < pair_end; step += 2) {
+            const Philox4 w = philox4x32_10(step >> 1, 0u, tid, 0u, H.key0, H.key1);
+            double g0, g1;
+            box_muller_f64(w.w[0], w.w[1], g0, g1);
+            if (one(step + 1, g0, w.w[2])) {
+                done = true;
+                last = step + 1;
+                break;
+            }
+            if (one(step + 2, g1, w.w[3])) {
+                done = true;
+                last = step + 2;
+                break;
+            }
+        }
+    }
+    if (!done && pair_end < W.s_end) {
+        const Philox4 w = philox4x32_10(pair_end >> 1, 0u, tid, 0u, H.key0, H.key1);
+        double g0, g1;
+        box_muller_f64(w.w[0], w.w[1], g0, g1);
+        if (one(pair_end + 1, g0, w.w[2])) {
+            done = true;
+            last = pair_end + 1;
         }
     }
     if (done) P.absorbed[slot] = last;
@@ -1101,6 +1180,7 @@ static void fill_set_dev(const ldpcp_sweep_set& s, uint64_t first, uint64_t star
     d.fS[1] = (float)d.S[1];
     d.fdtgm = (float)d.dtgm;
     d.fvar = (float)d.var;
+    d.fbm = (float)(-1.3862943611198906 * d.var * d.var);
     d.fminv_halfdt = (float)(d.minv * d.halfdt);
     d.fhalfdt = (float)d.halfdt;
     d.fexpmgdt = (float)d.expmgdt;
