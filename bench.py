@@ -10,21 +10,29 @@ ragged batch -- starting at rest in the well centre X0 = -2.9, Philox seed
 1000 + Eb index, step cap 1e8, absorbed trajectories retired after every
 launch.  A bench "step" is one launch (steps_per_launch integration steps of
 every live trajectory).  The FP32+compensated engine is timed (value) next to
-the plain FP64 engine, both with the autotuned launch shape.  Afterwards the
-Eb = 2 sets run to completion in both arithmetics and their mean first-passage
-times are compared (2^20 overdamped, 2^18 underdamped trajectories).
+the plain FP64 engine, both with the autotuned launch shape.  The roofline
+carries the census ops of the timed launches and, from a rocprofv3 --pmc pass
+run first, the VALU instructions the kernel actually issues.  Afterwards the
+single-barrier Eb = 2 and Eb = 3 sets and the double barrier's Eb = 2 sets run
+to completion in both arithmetics and their mean first-passage and transition
+path times are compared.
 
 Double barrier: 8 sets (Eb in {2,3,4,5} x {over, under}, sample_input/CuIDB_double
-geometry) of 2^22 trajectories, FP32 throughput and per-barrier transition
-paths.  Ragged: 256 parameter sets drawn from seed 7 (sweep.ragged_sets) run to
-completion in FP32 and FP64, per-set statistics compared.
+geometry) of 2^22 trajectories, FP32 throughput.  Ragged: 256 parameter sets
+drawn from seed 7 (sweep.ragged_sets) run to completion in FP32 and FP64,
+per-set statistics compared.
 
 Serving: a Poisson stream of sweep requests (20 per second for --serve-seconds,
-seed 11; single or double barrier, under- or overdamped, Eb ~ U[2, 6.5] with
-ratio 3.6, 1e3 to 1e6 trajectories log-uniform, dt = 1e-3, tau = 1e-4, cap
-5e4 steps) into the FP32 request server with a 200 GB HBM budget; p50 / p95 /
-p99 request latency and the aggregate trajectory-steps/s are reported next to
-the plain FP64 engine running the same requests one at a time.
+default 300, seed 11; single or double barrier, under- or overdamped,
+Eb ~ U[2, 6.5] with ratio 3.6, 1e3 to 1e6 trajectories log-uniform, dt = 1e-3,
+tau = 1e-4, cap 1e5 steps) into the FP32 request server with a 200 GB HBM
+budget; p50 / p95 / p99 request latency and the aggregate trajectory-steps/s
+are reported next to the plain FP64 engine running the same requests one at a
+time.
+
+CPU baseline: the reference's SIMD step loop restated with AVX-512 (or AVX2)
+intrinsics (oracle/cpu_simd.c), one thread per host core, on a bounded sample
+of the same 8 sets.
 """
 import argparse
 import json
@@ -94,25 +102,71 @@ def time_engine(precision, n_traj, steps, warmup, spl, tune=True):
     return res
 
 
-def mfpt_check(precision, n_traj):
-    """Eb = 2 sets (overdamped, underdamped) run to completion."""
+def mfpt_sets():
+    """The sets whose mean first-passage time is run to completion in both
+    arithmetics: single barrier Eb = 2 and 3, over- and underdamped (the timed
+    workload's geometry), and the double barrier's Eb = 2 sets (CuIDB_double
+    geometry); trajectory counts sized so that every set finishes (absorbed or
+    capped at 1e8 steps) within the bench's time."""
+    from langevindynamicsforpcp_amd.sweep import double_barrier_set
+    sets = []
+    for eb, over, n in ((2.0, True, 1 << 20), (2.0, False, 1 << 18), (3.0, True, 1 << 18), (3.0, False, 1 << 16)):
+        s = workload_sets(n, ebs=(eb,))[0 if over else 1]
+        sets.append(("single", s))
+    for over, n in ((True, 1 << 18), (False, 1 << 16)):
+        sets.append(("double", double_barrier_set(2.0, overdamped=over, n_traj=n, seed=1000)))
+    return sets
+
+
+def mfpt_check(precision, limit_s):
+    """Runs mfpt_sets() in one sweep until every trajectory is absorbed or
+    capped (or `limit_s` of wall time has passed); per set the MFPT, its
+    standard error, the mean transition-path time and, for the double barrier,
+    the mean transition-path time through each barrier."""
     from langevindynamicsforpcp_amd.sweep import Sweep
-    sets = workload_sets(n_traj, ebs=(2.0,))
-    # the underdamped set's MFPT is 15x the overdamped one's: a quarter of the
-    # trajectories keeps the run to completion short
-    sets[1].n_traj = max(n_traj // 4, 1)
+    named = mfpt_sets()
+    sets = [s for _, s in named]
     sw = Sweep(sets, precision=precision)
     sw.tune()
     t0 = time.perf_counter()
-    sw.run()
+    live = 1
+    while live and time.perf_counter() - t0 < limit_s:
+        live = sw.launch()
+    wall = time.perf_counter() - t0
     out = []
-    for i, s in enumerate(sets):
+    for i, (kind, s) in enumerate(named):
         m, e = sw.mfpt(i)
         st = sw.stats(i)
-        out.append({"overdamped": bool(s.overdamped), "mfpt": m, "stderr": e, "n_absorbed": st.n_absorbed,
-                    "n_capped": st.n_capped, "tpt_mean": st.tpt_sum / max(st.n_absorbed, 1)})
+        n = max(st.n_absorbed, 1)
+        r = {"model": kind, "eb": s.height[0][1] if kind == "single" else s.height[1][1],
+             "overdamped": bool(s.overdamped), "n_traj": s.n_traj, "mfpt": m, "stderr": e,
+             "n_absorbed": st.n_absorbed, "n_capped": st.n_capped,
+             "unfinished": s.n_traj - st.n_absorbed - st.n_capped, "tpt_mean": st.tpt_sum / n}
+        if kind == "double":
+            r["tpt_barrier0_mean"] = st.tptb_sum[0] / n
+            r["tpt_barrier1_mean"] = st.tptb_sum[1] / n
+        out.append(r)
     sw.close()
-    return out, time.perf_counter() - t0
+    return out, wall
+
+
+def compare_mfpt(f32, f64, t32, t64):
+    """Per set: FP32 MFPT (and TPT means) against FP64's, relative and in
+    units of the FP64 estimate's standard error."""
+    fin = {"sets": [], "seconds": {"f32": t32, "f64": t64}, "cap_steps": 10 ** 8}
+    for a, b in zip(f32, f64):
+        r = {"model": a["model"], "eb": a["eb"], "overdamped": a["overdamped"], "n_traj": a["n_traj"],
+             "mfpt_f32": a["mfpt"], "mfpt_f64": b["mfpt"], "rel_err": abs(a["mfpt"] - b["mfpt"]) / b["mfpt"],
+             "z": abs(a["mfpt"] - b["mfpt"]) / b["stderr"] if b["stderr"] > 0 else None,
+             "stderr_f64": b["stderr"], "tpt_f32": a["tpt_mean"], "tpt_f64": b["tpt_mean"],
+             "capped": [a["n_capped"], b["n_capped"]], "unfinished": [a["unfinished"], b["unfinished"]]}
+        if a["model"] == "double":
+            for k in ("tpt_barrier0_mean", "tpt_barrier1_mean"):
+                r[k] = [a[k], b[k]]
+        fin["sets"].append(r)
+    fin["max_rel_err"] = max(r["rel_err"] for r in fin["sets"])
+    fin["all_finished"] = all(r["unfinished"] == [0, 0] for r in fin["sets"])
+    return fin
 
 
 def double_barrier(n_traj, steps, warmup):
@@ -137,12 +191,10 @@ def double_barrier(n_traj, steps, warmup):
     sw.close()
     d = [b.traj_steps - a for a, b in zip(s0, st)]
     ops = sum(di * OPS_PER_STEP[(1, bool(x.overdamped))] for di, x in zip(d, sets)) / max(sum(d), 1)
-    per_set = []
-    for x, y in zip(sets, st):
-        n = max(y.n_absorbed, 1)
-        per_set.append({"eb": x.height[0][1], "overdamped": bool(x.overdamped), "absorbed": y.n_absorbed,
-                        "mean_fpt_absorbed": y.fpt_sum / n, "mean_tpt": y.tpt_sum / n,
-                        "mean_tpt_barrier0": y.tptb_sum[0] / n, "mean_tpt_barrier1": y.tptb_sum[1] / n})
+    # statistics of this leg would be truncated after a few launches: the
+    # double barrier's completed MFPT and per-barrier TPT are in the mfpt leg
+    per_set = [{"eb": x.height[1][1], "overdamped": bool(x.overdamped), "absorbed_so_far": y.n_absorbed}
+               for x, y in zip(sets, st)]
     return {"n_traj_per_set": n_traj, "sets": len(sets), "traj_steps_per_s": sum(d) / wall, "wall_s": wall,
             "ops_per_traj_step": ops, "launch": {"block": c.block, "blocks_per_cu": c.blocks_per_cu,
                                                   "steps_per_launch": c.steps_per_launch},
@@ -190,10 +242,20 @@ def ragged(max_steps=200000, seed=7):
                               "max_z_mfpt": float(max(z_mfpt)) if z_mfpt else None}}
 
 
-def serving(seconds, budget_gb, rate=20.0, seed=11, fp64_limit_s=60.0):
+# Requests of the serving stream: dt and tau of the C2 sets scaled by 10
+# (same dt/tau = 10), a step cap of 1e5 steps = 100 time units.  At 20
+# requests/s and 1.45e5 trajectories per request on average (log-uniform in
+# [1e3, 1e6]) the stream offers 2.9e6 trajectories/s: a cap of 1e5 steps keeps
+# the offered work (<= 2.9e11 trajectory-steps/s) below the FP32 engine's rate,
+# where C2's cap of 1e8 steps would offer ~400x what the device can do, and
+# the latency percentiles would then measure an ever-growing queue.
+SERVE_DT, SERVE_TAU, SERVE_CAP = 1e-3, 1e-4, 100000
+
+
+def serving(seconds, budget_gb, rate=20.0, seed=11, fp64_limit_s=30.0, engine_rate=None):
     import numpy as np
     from langevindynamicsforpcp_amd import server as S
-    reqs = S.poisson_requests(rate, seconds, seed)
+    reqs = S.poisson_requests(rate, seconds, seed, max_steps=SERVE_CAP, timestep=SERVE_DT, tau=SERVE_TAU)
     srv = S.Server(budget_bytes=int(budget_gb * 1e9), precision=32)
     # warm the graphs and kernels on a throwaway request before the clock starts
     srv.submit(reqs[0][1])
@@ -213,7 +275,16 @@ def serving(seconds, budget_gb, rate=20.0, seed=11, fp64_limit_s=60.0):
     out = {"requests": len(reqs), "rate_per_s": rate, "duration_s": seconds, "seed": seed,
            "budget_gb": budget_gb, "wall_s": res["wall"], "traj_steps": steps,
            "traj_steps_per_s": steps / res["wall"], "latency_s": S.percentiles(res["latency"]),
-           "latency_mean_s": float(np.mean(res["latency"])), "left_resident": info["resident"]}
+           "latency_mean_s": float(np.mean(res["latency"])), "left_resident": info["resident"],
+           "request": {"timestep": SERVE_DT, "tau": SERVE_TAU, "max_steps": SERVE_CAP,
+                       "trajectories": "log-uniform in [1e3, 1e6]", "eb": "U[2, 6.5], ratio 3.6",
+                       "cap_note": "cap 1e5 steps (100 time units at dt 1e-3): the stream offers at most "
+                                   "2.9e6 trajectories/s x 1e5 steps = 2.9e11 trajectory-steps/s, under the FP32 "
+                                   "engine's rate; C2's 1e8 cap would offer ~400x the device's rate"},
+           "absorbed_fraction": float(sum(r.n_absorbed for r in res["results"]) /
+                                      max(sum(r.n_absorbed + r.n_capped for r in res["results"]), 1))}
+    if engine_rate:
+        out["served_over_engine_peak"] = out["traj_steps_per_s"] / engine_rate
     # the plain FP64 engine, one job at a time, on the requests its time limit allows
     S.one_at_a_time(reqs[:1], precision=64)  # warm-up
     one = S.one_at_a_time(reqs, precision=64, limit_s=fp64_limit_s)
@@ -257,43 +328,86 @@ def run_mode_c1():
                               "7.13 min); it publishes no figure for this case"}
 
 
-def cpu_baseline(seconds=12.0):
-    """The CPU restatement (oracle, lane mode, same Philox noise and step),
-    one process per host core, on a bounded sample of the same workload: the
-    workers cycle over the 8 parameter sets, trajectories capped at 2e5 steps."""
-    ncores = max(1, min(16, os.cpu_count() or 1))
-    steps = 200000
-    code = (
-        "import sys, time, ctypes as C; sys.path.insert(0, %r); import oracle_lib as ol\n"
-        "lib = ol.load(); w = int(sys.argv[1]); eb = (2.0, 3.0, 4.0, 5.0)[w %% 4]; over = (w // 4) %% 2 == 0\n"
-        "p = ol.make_params(model=0, overdamped=int(over), x0=-2.9, v0=0.0, randomize_v=0, periodic_length=5.8, "
-        "heights=[[3.6 * eb, eb], [3.6 * eb, eb]], half_width=[0.65, 0.65], timestep=1e-4, tau=1e-5, nsteps=%d, "
-        "save_freq=%d, b_waiting=0, b_fpt=0, b_tpt=0, b_fptcg=0, absorb=1)\n"
-        "t0 = time.perf_counter(); n = 0; k = w\n"
-        "while time.perf_counter() - t0 < %f:\n"
-        "    out = ol.Lane(); lib.ldo_lane_run_philox(C.byref(p), C.byref(out), 1000 + w %% 4, k, 0, 0, None, 0)\n"
-        "    n += out.steps_done; k += %d\n"
-        "print(n, time.perf_counter() - t0)\n"
-    ) % (os.path.join(REPO, "tests"), steps, steps, seconds, ncores)
-    procs = [subprocess.Popen([sys.executable, "-c", code, str(i)], stdout=subprocess.PIPE, text=True)
-             for i in range(ncores)]
-    total, secs = 0, 0.0
-    for pr in procs:
-        out, _ = pr.communicate()
-        n, s = out.split()
-        total += int(n)
-        secs = max(secs, float(s))
-    return {"value": total / secs, "unit": "traj-steps/s", "cores": ncores, "kind": "port",
-            "sample": f"oracle lane mode (scalar C, FP64, Philox noise), one process per core for ~{seconds:.0f} s, "
-                      f"workers cycling over the 8 parameter sets, trajectories absorbed or capped at {steps} steps"}
+def host_cores():
+    """Host cores this process may use: the affinity mask, the cgroup CPU
+    quota where one is set, and at most 16 (one GPU's share of a shared host)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(float(quota) / float(period))))
+    except (OSError, ValueError):
+        pass
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(seconds=10.0, cap=200000):
+    """The reference's SIMD CPU path restated (oracle/cpu_simd.c: W = 8 AVX-512
+    or W = 4 AVX2 lanes per vector, the reference's step, telegraph and
+    hand-over loop, Philox noise), one thread per host core, on a bounded
+    sample of the same workload: the 8 parameter sets, threads spread over
+    them, trajectories absorbed or capped at `cap` steps, ~`seconds` of wall
+    time."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib as ol
+    lib = ol.load_simd()
+    threads = host_cores()
+    params, seeds = [], []
+    for s in workload_sets(1):
+        params.append(ol.make_params(model=0, overdamped=int(s.overdamped), x0=s.x0, v0=s.v0, randomize_v=0,
+                                     periodic_length=s.periodic_length,
+                                     heights=[[s.height[0][0], s.height[0][1]]] * 2,
+                                     half_width=[s.half_width[0], s.half_width[1]], timestep=s.timestep, tau=s.tau,
+                                     nsteps=cap, save_freq=cap, b_waiting=0, b_fpt=0, b_tpt=0, b_fptcg=0, absorb=1))
+        seeds.append(s.seed)
+    _, _, steps, done, wall = ol.simd_absorb(lib, params, seeds, seconds=seconds, threads=threads,
+                                             want_paths=False)
+    width = lib.ldo_simd_width()
+    return {"value": steps / wall, "unit": "traj-steps/s", "cores": threads, "kind": "port",
+            "per_core": steps / wall / threads, "simd_width": width,
+            "isa": "AVX-512" if width == 8 else "AVX2+FMA", "cpu_model": cpu_model(),
+            "host_cpus_visible": os.cpu_count(), "trajectories_finished": done,
+            "sample": f"the reference's SIMD step loop restated (oracle/cpu_simd.c, FP64, {width} lanes per vector, "
+                      f"Philox + IEEE Box-Muller noise, bit-exact with the oracle per trajectory), {threads} threads "
+                      f"for {wall:.1f} s over the 8 parameter sets, trajectories absorbed or capped at {cap} steps",
+            "reference_published_per_core": 2.8e7}
 
 
 def pmc_child(args):
-    """Counter-run body: a short run of the timed workload, then a plain
-    device-to-device copy of known size (the runtime's copy kernel) that
-    calibrates the counters."""
+    """Counter-run body: four launches of the timed workload (FP32 engine, the
+    default launch shape with --steps-per-launch steps), their trajectory-step
+    counts written to --pmc-out, then a plain device-to-device copy of known
+    size (the runtime's copy kernel) that calibrates the traffic counters."""
     import ctypes as C
-    time_engine(32, args.n_traj, 3, 1, args.steps_per_launch, tune=False)
+    from langevindynamicsforpcp_amd.sweep import Sweep
+    sets = workload_sets(args.n_traj)
+    sw = Sweep(sets, precision=32)
+    c = sw.config()
+    sw.set_config(block=c.block, blocks_per_cu=c.blocks_per_cu, steps_per_launch=args.steps_per_launch,
+                  use_graph=c.use_graph)
+    per_launch, prev = [], total_steps(sw, len(sets))
+    for _ in range(4):
+        sw.launch()
+        cur = total_steps(sw, len(sets))
+        per_launch.append(cur - prev)
+        prev = cur
+    c = sw.config()
+    sw.close()
+    if args.pmc_out:
+        with open(args.pmc_out, "w") as f:
+            json.dump({"traj_steps_per_launch": per_launch,
+                       "launch": {"block": c.block, "blocks_per_cu": c.blocks_per_cu,
+                                  "steps_per_launch": c.steps_per_launch}}, f)
     hip = None
     for name in ("libamdhip64.so", "libamdhip64.so.7", "/opt/rocm/lib/libamdhip64.so"):
         try:
@@ -334,37 +448,64 @@ def read_counter_csv(out_dir, counter):
     return out
 
 
-def pmc_traffic(args, timeout=240):
-    """HBM bytes per launch of the FP32 step kernel from two rocprofv3 --pmc runs
-    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
-    counter reads for a plain copy of known size.  None if rocprofv3 is missing
-    or a run fails."""
+def run_pmc(args, counters, name, timeout=240):
+    """One rocprofv3 --pmc pass of the counter-run body; returns
+    ({counter: {kernel: [per-dispatch values]}}, the body's launch record), or
+    None if rocprofv3 is missing or the pass fails.  The raw csv stays under
+    --pmc-dir when one is given."""
     import shutil
     import tempfile
     prof = shutil.which("rocprofv3")
     if prof is None:
         return None
+    keep = bool(args.pmc_dir)
+    d = os.path.join(args.pmc_dir, name) if keep else tempfile.mkdtemp(prefix="ldpcp_pmc_")
+    os.makedirs(d, exist_ok=True)
+    rec = os.path.join(d, "launches.json")
+    cmd = [prof, "--pmc", *counters, "--kernel-include-regex", "engine_step|copyBuffer", "-d", d, "-o", "pmc",
+           "-f", "csv", "--", sys.executable, os.path.abspath(__file__), "--pmc-child", "--pmc-out", rec,
+           "--n-traj", str(args.n_traj), "--steps-per-launch", str(args.steps_per_launch or STEPS_PER_LAUNCH)]
+    try:
+        pr = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=timeout,
+                            check=False)
+    except subprocess.TimeoutExpired:
+        print(f"# pmc {name}: run timed out", file=sys.stderr, flush=True)
+        return None
+    vals = {c: read_counter_csv(d, c) for c in counters}
+    try:
+        launches = json.load(open(rec))
+    except (OSError, ValueError):
+        launches = None
+    if not keep:
+        shutil.rmtree(d, ignore_errors=True)
+    n_step = [len([v for k, vs in vals[c].items() if "engine_step" in k for v in vs]) for c in counters]
+    if launches is None or min(n_step) < 4:
+        print(f"# pmc {name}: {n_step} step dispatches (rc {pr.returncode}); output tail:\n{pr.stdout[-1500:]}",
+              file=sys.stderr, flush=True)
+        return None
+    return vals, launches
+
+
+def step_values(vals, counter):
+    """The engine step kernel's per-dispatch values of one counter."""
+    return [v for k, vs in vals[counter].items() if "engine_step" in k for v in vs]
+
+
+def pmc_traffic(args):
+    """HBM bytes per launch of the FP32 step kernel from two rocprofv3 --pmc runs
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same
+    counter reads for a plain copy of known size.  None if a run fails."""
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        d = tempfile.mkdtemp(prefix="ldpcp_pmc_")
-        cmd = [prof, "--pmc", counter, "--kernel-include-regex", "engine_step|copyBuffer", "-d", d, "-o", "pmc",
-               "-f", "csv", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
-               "--n-traj", str(args.n_traj), "--steps-per-launch", str(args.steps_per_launch)]
-        try:
-            pr = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=timeout,
-                                check=False)
-        except subprocess.TimeoutExpired:
-            print(f"# traffic: {counter} run timed out", file=sys.stderr, flush=True)
+        r = run_pmc(args, [counter], counter.lower())
+        if r is None:
             return None
-        vals = read_counter_csv(d, counter)
-        shutil.rmtree(d, ignore_errors=True)
-        step = [v for k, vs in vals.items() if "engine_step" in k for v in vs]
+        vals, launches = r
+        step = step_values(vals, counter)
         # the runtime's copy kernel also serves the engine's small host copies:
         # the calibration copies are the last three dispatches
-        copy = [v for k, vs in vals.items() if "copyBuffer" in k for v in vs][-3:]
-        if len(step) < 3 or not copy:
-            print(f"# traffic: {counter} run gave {len(step)} step and {len(copy)} copy dispatches (rc {pr.returncode}); "
-                  f"kernels {sorted(vals)[:6]}; output tail:\n{pr.stdout[-1500:]}", file=sys.stderr, flush=True)
+        copy = [v for k, vs in vals[counter].items() if "copyBuffer" in k for v in vs][-3:]
+        if not copy:
             return None
         factor = (COPY_BYTES / 1024.0) / (sum(copy) / len(copy))
         res[counter] = {"kib_per_launch": sum(step[-3:]) / 3, "copy_factor": factor,
@@ -378,6 +519,31 @@ def pmc_traffic(args, timeout=240):
     return res
 
 
+VALU_COUNTERS = ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_VALU2",
+                 "SQ_THREAD_CYCLES_VALU", "SQ_BUSY_CYCLES", "GRBM_GUI_ACTIVE")
+
+
+def pmc_valu(args):
+    """Instruction counts of the FP32 step kernel (one rocprofv3 --pmc pass):
+    VALU and SALU wave-instructions per wavefront-step (64 trajectory-steps),
+    the share of VALU issue cycles in which a second VALU instruction was
+    active, and the share of lanes active, over the last three of four
+    launches.  None if the pass fails."""
+    r = run_pmc(args, list(VALU_COUNTERS), "valu")
+    if r is None:
+        return None
+    vals, launches = r
+    steps = sum(launches["traj_steps_per_launch"][-3:])
+    tot = {c: sum(step_values(vals, c)[-3:]) for c in VALU_COUNTERS}
+    wave_steps = steps / 64.0
+    return {"valu_insts_per_wave_step": tot["SQ_INSTS_VALU"] / wave_steps,
+            "salu_insts_per_wave_step": tot["SQ_INSTS_SALU"] / wave_steps,
+            "valu2_over_valu_active": tot["SQ_ACTIVE_INST_VALU2"] / max(tot["SQ_ACTIVE_INST_VALU"], 1.0),
+            "lanes_active": tot["SQ_THREAD_CYCLES_VALU"] / max(64.0 * tot["SQ_ACTIVE_INST_VALU"], 1.0),
+            "traj_steps_counted": steps, "launch": launches["launch"],
+            "per_dispatch": {c: step_values(vals, c)[-3:] for c in VALU_COUNTERS}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -385,7 +551,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--steps-per-launch", type=int, default=0, help="0 = the autotuner's choice")
     ap.add_argument("--n-traj", type=int, default=1 << 20, help="trajectories per parameter set")
-    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-seconds", type=float, default=300.0)
+    ap.add_argument("--serve-fp64-seconds", type=float, default=30.0,
+                    help="service time given to the one-at-a-time FP64 comparison")
+    ap.add_argument("--mfpt-limit", type=float, default=120.0,
+                    help="wall-time limit of each arithmetic's MFPT run to completion")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-finish", action="store_true", help="skip the MFPT check")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of the autotuner's")
@@ -396,6 +566,9 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs")
     ap.add_argument("--no-run-mode", action="store_true", help="skip the drop-in executable's 32-trajectory case")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-out", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-dir", default="", help="keep the counter passes' raw csv under this directory")
+    ap.add_argument("--no-valu", action="store_true", help="skip the rocprofv3 --pmc instruction-count pass")
     args = ap.parse_args()
     if args.pmc_child:
         if not args.steps_per_launch:
@@ -411,6 +584,8 @@ def main():
     # counter runs first, while this process has not opened the GPU
     traffic = None if args.no_traffic else pmc_traffic(args)
     print(f"# traffic {traffic}", file=sys.stderr, flush=True)
+    valu = None if args.no_valu else pmc_valu(args)
+    print(f"# valu {valu}", file=sys.stderr, flush=True)
 
     from langevindynamicsforpcp_amd import _native
     lib = _native.load()
@@ -431,15 +606,10 @@ def main():
 
     fin = None
     if not args.no_finish:
-        f32, t32 = mfpt_check(32, args.n_traj)
-        f64, t64 = mfpt_check(64, args.n_traj)
-        fin = {"sets": [], "seconds": {"f32": t32, "f64": t64}}
-        for a, b in zip(f32, f64):
-            fin["sets"].append({"overdamped": a["overdamped"], "mfpt_f32": a["mfpt"], "mfpt_f64": b["mfpt"],
-                                "rel_err": abs(a["mfpt"] - b["mfpt"]) / b["mfpt"], "stderr_f64": b["stderr"],
-                                "tpt_f32": a["tpt_mean"], "tpt_f64": b["tpt_mean"],
-                                "capped": [a["n_capped"], b["n_capped"]]})
-        fin["max_rel_err"] = max(s["rel_err"] for s in fin["sets"])
+        f32, t32 = mfpt_check(32, args.mfpt_limit)
+        print(f"# mfpt f32 {t32:.1f} s", file=sys.stderr, flush=True)
+        f64, t64 = mfpt_check(64, args.mfpt_limit)
+        fin = compare_mfpt(f32, f64, t32, t64)
         print(f"# mfpt {fin}", file=sys.stderr, flush=True)
 
     dbl = None
@@ -458,12 +628,29 @@ def main():
 
     serve = None
     if not args.no_serve:
-        serve = serving(args.serve_seconds, args.serve_budget_gb)
+        serve = serving(args.serve_seconds, args.serve_budget_gb, fp64_limit_s=args.serve_fp64_seconds,
+                        engine_rate=v32)
         print(f"# serving {serve}", file=sys.stderr, flush=True)
 
     if not args.steps_per_launch:
         args.steps_per_launch = r32["config"]["steps_per_launch"]
     ops = r32["ops_per_step"]
+    issued = None
+    if valu:
+        # one wave64 VALU instruction per SIMD per 4 cycles is single issue
+        single = ncu.value * 4 * 2.4e9 / 4
+        rate = k32 / 64.0 * valu["valu_insts_per_wave_step"]
+        issued = {"valu_insts_per_wave_step": valu["valu_insts_per_wave_step"],
+                  "salu_insts_per_wave_step": valu["salu_insts_per_wave_step"],
+                  "valu_wave_insts_per_s": rate, "frac_of_single_issue": rate / single,
+                  "frac_of_co_issue": rate / (2.0 * single),
+                  "valu2_over_valu_active": valu["valu2_over_valu_active"], "lanes_active": valu["lanes_active"],
+                  "census_ops_per_issued_valu": ops / valu["valu_insts_per_wave_step"],
+                  "counter_launch": valu["launch"], "counters": list(VALU_COUNTERS),
+                  "note": "rocprofv3 --pmc pass of the same FP32 engine (default launch shape, last 3 of 4 "
+                          "launches) before the GPU is opened here: VALU wave-instructions per wavefront-step "
+                          "(64 trajectory-steps) times the timed kernel rate; single issue = CUs x 4 SIMDs x "
+                          "2.4 GHz / 4 cycles per wave64 instruction, co-issue = twice that"}
     achieved = k32 * ops / 1e12
     peak = P_VALU / 1e12
     line = {
@@ -496,6 +683,7 @@ def main():
                      "avg_launch_us": 1e3 * r32["kernel_ms"] / max(r32["launches"], 1),
                      "traj_steps_per_launch": r32["traj_steps"] / max(r32["launches"], 1),
                      "ops_per_traj_step": ops,
+                     "issued": issued,
                      "note": "VALU bound (no MFMA; HBM traffic per trajectory-step ~0). achieved = census VALU "
                              "lane-ops per trajectory-step (FMA = 1 op; Philox 40, Box-Muller 12, flip 2, rest "
                              "39-72 by kind) x trajectory-steps / launch time (HIP events around each launch: "
