@@ -538,7 +538,7 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
     // after an odd step and after it when the window ends on one.
     uint32_t begin = W.s0;
     if ((begin & 1u) && begin < W.s_end) {
-        const Philox4 w = philox4x32_10(begin >> 1, 0u, tid, 0u, H.key0, H.key1);
+        const Philox4 w = philox4x32_10_lane_ctr(begin >> 1, 0u, tid, 0u, H.key0, H.key1);
         float g0, g1;
         box_muller_f32_scaled(w.w[0], w.w[1], H.fbm, g0, g1);
         ++begin;
@@ -550,7 +550,7 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
     const uint32_t pair_end = begin + ((W.s_end - begin) & ~1u);
     if (!done) {
         for (uint32_t step = begin; step < pair_end; step += 2) {
-            const Philox4 w = philox4x32_10(step >> 1, 0u, tid, 0u, H.key0, H.key1);
+            const Philox4 w = philox4x32_10_lane_ctr(step >> 1, 0u, tid, 0u, H.key0, H.key1);
             float g0, g1;
             box_muller_f32_scaled(w.w[0], w.w[1], H.fbm, g0, g1);
             if (one(step + 1, g0, w.w[2])) {
@@ -566,7 +566,7 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
         }
     }
     if (!done && pair_end < W.s_end) {
-        const Philox4 w = philox4x32_10(pair_end >> 1, 0u, tid, 0u, H.key0, H.key1);
+        const Philox4 w = philox4x32_10_lane_ctr(pair_end >> 1, 0u, tid, 0u, H.key0, H.key1);
         float g0, g1;
         box_muller_f32_scaled(w.w[0], w.w[1], H.fbm, g0, g1);
         if (one(pair_end + 1, g0, w.w[2])) {
