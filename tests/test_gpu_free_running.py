@@ -11,11 +11,14 @@ different realisations of the same process, so their statistics must agree:
   p > 0.01, and the MFPTs differ by less than 1.96 combined standard errors
   (the GPU's MFPT lies in the oracle's 95 % confidence interval, widened by
   the GPU estimate's own error);
-* the second-neighbour first passages (FPT2, :2413-2489) and the
-  transition-path times (TPT1, :2491-2530) pass the same KS test, and their
-  means differ by less than 3 combined standard errors (secondary statistics
-  of the same runs);
-* the MSD over the saved frames agrees within 3 standard errors at every lag;
+* the transition-path times (TPT1, :2491-2530) pass the same KS test and
+  their means lie within the same 95 % bound;
+* the second-neighbour first passages (FPT2, :2413-2489) pass the KS test and
+  their means differ by less than 3 combined standard errors (a secondary
+  statistic of the same runs);
+* the MSD over the saved frames agrees within 3 standard errors at every lag
+  the reference's runner evaluates (GenerateLog10Scale, MOLUTILITY.cpp:59-73,
+  20 lags per decade over the 2001 frames);
 * each lane's fraction of steps with the barrier in state 1 sits at the
   telegraph's stationary 1/2 (EQ true): their mean is within 4 standard
   errors of it.
@@ -25,6 +28,7 @@ and the first FPTCG list the secondary one.  These are the north-star's
 free-running criteria (the reference's 95 % CI and a KS p > 0.01), with the
 oracle standing in for the reference program.
 """
+import ctypes as C
 import math
 
 import numpy as np
@@ -51,7 +55,7 @@ def _compare(a, b, what, z_max):
 # lists compared per model: (name, GPU event list, oracle capture key and list index, z bound);
 # the double barrier keeps FPT2 and FPTCG only (Double.h:2974-3006, 3103-3154)
 COMPARED = {
-    0: [("fpt", 0, ("fpt", None), 1.96), ("fpt2", 1, ("fpt2", None), 3.0), ("tpt", 2, ("tpt", None), 3.0)],
+    0: [("fpt", 0, ("fpt", None), 1.96), ("fpt2", 1, ("fpt2", None), 3.0), ("tpt", 2, ("tpt", None), 1.96)],
     1: [("fpt2", 1, ("fpt2", None), 1.96), ("fptcg0", 6, ("cg", 0), 3.0)],
 }
 
@@ -63,7 +67,7 @@ def _lane_count(lane, key, j):
 @pytest.mark.parametrize("name", ["single_over", "single_under", "double_over", "double_under"])
 def test_gpu_free_running_statistics_match_oracle(oracle, name):
     nsteps, capn, W, ranks, n_gpu = 200000, 8192, 16, 32, 4096
-    p = make(name, nsteps=nsteps, save_freq=1000)
+    p = make(name, nsteps=nsteps, save_freq=100)
     lists = COMPARED[p.model]
     print(f"[free running] {name}: oracle {ranks} x {W} lanes, GPU {n_gpu} trajectories, {nsteps} steps", flush=True)
     ref = {k: [] for k, *_ in lists}
@@ -88,16 +92,23 @@ def test_gpu_free_running_statistics_match_oracle(oracle, name):
 
     # MSD per trajectory over the saved frames (MeanSquareDisplacementBruteForce,
     # Single.h:24-45): the mean over trajectories within 3 standard errors at
-    # every lag
-    lags = np.array([1, 2, 5, 10, 20, 50, 100], np.uint64)
+    # every lag of the runner's log10 scale (lag 0 excluded)
+    n_frames = ref_x.shape[1]
+    buf = np.zeros(512, np.uint64)
+    nl = oracle.ldo_log10_scale(p.timestep * p.save_freq, n_frames, 1000, 20, 0,
+                                buf.ctypes.data_as(C.POINTER(C.c_uint64)), len(buf))
+    lags = buf[:nl]
+    assert nl > 40 and lags[0] == 1 and lags[-1] <= n_frames - 1
     m_gpu = le.msd(lags)
     m_ref = np.array([[np.mean((x[l:] - x[:-l]) ** 2) for l in lags.astype(int)] for x in ref_x])
+    zs = []
     for k, l in enumerate(lags):
         a, b = m_gpu[:, k], m_ref[:, k]
         se = math.sqrt(a.var() / len(a) + b.var() / len(b))
-        print(f"[free running] msd lag {int(l)}: {a.mean():.4f} / {b.mean():.4f}, z {(a.mean() - b.mean()) / se:+.2f}",
-              flush=True)
-        assert abs(a.mean() - b.mean()) < 3 * se, int(l)
+        zs.append((a.mean() - b.mean()) / se)
+    worst = int(np.argmax(np.abs(zs)))
+    print(f"[free running] msd at {nl} lags: max |z| {abs(zs[worst]):.2f} at lag {int(lags[worst])}", flush=True)
+    assert max(abs(z) for z in zs) < 3.0, (int(lags[worst]), zs[worst])
 
     # telegraph state: fraction of counted steps in state 1 (EQ true: 1/2)
     lanes = le.lanes()
