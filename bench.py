@@ -59,14 +59,8 @@ COPY_BYTES = 1 << 28  # 256 MiB read + 256 MiB written per calibration copy
 
 
 def workload_sets(n_traj, ebs=EBS):
-    from langevindynamicsforpcp_amd.sweep import sweep_set
-    out = []
-    for over in (True, False):
-        for k, eb in enumerate(ebs):
-            out.append(sweep_set(model=0, overdamped=over, eb=eb, ratio=3.6, n_traj=n_traj, max_steps=10 ** 8,
-                                 seed=1000 + EBS.index(eb), x0=-2.9, periodic_length=5.8, half_width=(0.65, 0.65),
-                                 timestep=1e-4, tau=1e-5, randomize_v=False, v0=0.0))
-    return out
+    from langevindynamicsforpcp_amd import workloads as WL
+    return WL.single_barrier(n_traj=n_traj, ebs=ebs)
 
 
 def total_steps(sw, n):
@@ -108,13 +102,13 @@ def mfpt_sets():
     workload's geometry), and the double barrier's Eb = 2 sets (CuIDB_double
     geometry); trajectory counts sized so that every set finishes (absorbed or
     capped at 1e8 steps) within the bench's time."""
-    from langevindynamicsforpcp_amd.sweep import double_barrier_set
+    from langevindynamicsforpcp_amd import workloads as WL
     sets = []
     for eb, over, n in ((2.0, True, 1 << 20), (2.0, False, 1 << 18), (3.0, True, 1 << 18), (3.0, False, 1 << 16)):
         s = workload_sets(n, ebs=(eb,))[0 if over else 1]
         sets.append(("single", s))
     for over, n in ((True, 1 << 18), (False, 1 << 16)):
-        sets.append(("double", double_barrier_set(2.0, overdamped=over, n_traj=n, seed=1000)))
+        sets.append(("double", WL.double_barrier(n_traj=n, ebs=(2.0,), overdamped=(over,))[0]))
     return sets
 
 
@@ -175,9 +169,9 @@ def double_barrier(n_traj, steps, warmup):
     x {overdamped, underdamped}, n_traj each, FP32: throughput over `steps`
     launches after the autotuner and `warmup` launches, and the per-barrier
     transition paths of the trajectories absorbed by then."""
-    from langevindynamicsforpcp_amd.sweep import Sweep, double_barrier_set
-    sets = [double_barrier_set(eb, overdamped=over, n_traj=n_traj, seed=1000 + k)
-            for over in (True, False) for k, eb in enumerate(EBS)]
+    from langevindynamicsforpcp_amd import workloads as WL
+    from langevindynamicsforpcp_amd.sweep import Sweep
+    sets = WL.double_barrier(n_traj=n_traj)
     sw = Sweep(sets, precision=32)
     c = sw.tune()
     for _ in range(warmup):
@@ -242,20 +236,21 @@ def ragged(max_steps=200000, seed=7):
                               "max_z_mfpt": float(max(z_mfpt)) if z_mfpt else None}}
 
 
-# Requests of the serving stream: dt and tau of the C2 sets scaled by 10
-# (same dt/tau = 10), a step cap of 1e5 steps = 100 time units.  At 20
-# requests/s and 1.45e5 trajectories per request on average (log-uniform in
-# [1e3, 1e6]) the stream offers 2.9e6 trajectories/s: a cap of 1e5 steps keeps
-# the offered work (<= 2.9e11 trajectory-steps/s) below the FP32 engine's rate,
-# where C2's cap of 1e8 steps would offer ~400x what the device can do, and
-# the latency percentiles would then measure an ever-growing queue.
-SERVE_DT, SERVE_TAU, SERVE_CAP = 1e-3, 1e-4, 100000
+# Requests of the serving stream (workloads.serving_requests): dt and tau of
+# the single-barrier sets scaled by 10 (same dt/tau = 10), a step cap of 1e5
+# steps = 100 time units.  At 20 requests/s and 1.45e5 trajectories per request
+# on average (log-uniform in [1e3, 1e6]) the stream offers 2.9e6
+# trajectories/s: a cap of 1e5 steps keeps the offered work (<= 2.9e11
+# trajectory-steps/s) below the FP32 engine's rate, where the sweep's cap of
+# 1e8 steps would offer ~400x what the device can do, and the latency
+# percentiles would then measure an ever-growing queue.
 
 
 def serving(seconds, budget_gb, rate=20.0, seed=11, fp64_limit_s=30.0, engine_rate=None):
     import numpy as np
     from langevindynamicsforpcp_amd import server as S
-    reqs = S.poisson_requests(rate, seconds, seed, max_steps=SERVE_CAP, timestep=SERVE_DT, tau=SERVE_TAU)
+    from langevindynamicsforpcp_amd import workloads as WL
+    reqs = WL.serving_requests(seconds, rate=rate, seed=seed)
     srv = S.Server(budget_bytes=int(budget_gb * 1e9), precision=32)
     # warm the graphs and kernels on a throwaway request before the clock starts
     srv.submit(reqs[0][1])
@@ -276,7 +271,7 @@ def serving(seconds, budget_gb, rate=20.0, seed=11, fp64_limit_s=30.0, engine_ra
            "budget_gb": budget_gb, "wall_s": res["wall"], "traj_steps": steps,
            "traj_steps_per_s": steps / res["wall"], "latency_s": S.percentiles(res["latency"]),
            "latency_mean_s": float(np.mean(res["latency"])), "left_resident": info["resident"],
-           "request": {"timestep": SERVE_DT, "tau": SERVE_TAU, "max_steps": SERVE_CAP,
+           "request": {"timestep": WL.SERVING_TIMESTEP, "tau": WL.SERVING_TAU, "max_steps": WL.SERVING_CAP,
                        "trajectories": "log-uniform in [1e3, 1e6]", "eb": "U[2, 6.5], ratio 3.6",
                        "cap_note": "cap 1e5 steps (100 time units at dt 1e-3): the stream offers at most "
                                    "2.9e6 trajectories/s x 1e5 steps = 2.9e11 trajectory-steps/s, under the FP32 "
