@@ -321,16 +321,26 @@ template <typename Real, int MODEL>
 __device__ __forceinline__ int handover(const Hot& H, const SetDev* __restrict__ Sg, uint32_t& flags, uint32_t* ts,
                                         int kp, int kc, int state, uint32_t step, uint32_t tid, uint32_t& n_catch)
 {
+    // the single barrier uses slots 0 and 1 only (one per cell): two-way
+    // selects instead of four-way ones for the lane's slot index
+    auto ts_set = [&](int k, uint32_t v) __attribute__((always_inline)) {
+        if (MODEL == 0) {
+            ts[0] = k == 0 ? v : ts[0];
+            ts[1] = k == 1 ? v : ts[1];
+        } else {
+            ts[k] = v;
+        }
+    };
     if (kp >= 0) {
         flags = (flags & ~(F_SLOT_STATE << kp)) | (F_SLOT_VALID << kp) | (state ? (F_SLOT_STATE << kp) : 0u);
-        ts[kp] = step;
+        ts_set(kp, step);
     }
     if (kc < 0) return state;  // leaving the two-cell window: absorbed this step
     int prev;
     uint32_t ts_c;
     if (flags & (F_SLOT_VALID << kc)) {
         prev = (flags >> (8 + kc)) & 1;
-        ts_c = ts[kc];
+        ts_c = MODEL == 0 ? (kc == 0 ? ts[0] : ts[1]) : ts[kc];
     } else {
         // first touch: the state the barrier was created with, at step 0
         const int array = MODEL == 0 ? 0 : (kc & 1);
@@ -339,7 +349,7 @@ __device__ __forceinline__ int handover(const Hot& H, const SetDev* __restrict__
         prev = below(w.w[0], Sg->T_create) ? 0 : 1;
         ts_c = 0;
         flags |= (F_SLOT_VALID << kc) | (prev ? (F_SLOT_STATE << kc) : 0u);
-        ts[kc] = 0;
+        ts_set(kc, 0u);
     }
     if (H.tau_zero) return prev;
     const Philox4 w = philox4x32_10(n_catch, 0u, tid, 2u, H.key0, H.key1);
@@ -357,7 +367,8 @@ __device__ __forceinline__ int handover(const Hot& H, const SetDev* __restrict__
         const double scale = H.equal_prob ? 0.5 : (prev == 0 ? K / (1.0 + K) : 1.0 / (1.0 + K));
         pc = (Real)(scale * (1.0 - e));
     }
-    const bool flip = (Real)unit_f64(w.w[0]) < pc;
+    // the catch-up uniform in the stepping precision
+    const bool flip = (sizeof(Real) == 4 ? (Real)unit_f32(w.w[0]) : (Real)unit_f64(w.w[0])) < pc;
     return flip ? (prev ^ 1) : prev;
 }
 
