@@ -12,7 +12,8 @@
 //           words 2,3 -> per-step flip uniforms (odd, even step)
 //   role 1: initial draws, index 0: words 0,1 -> Box-Muller (cos -> V0, sin -> R0),
 //           word 2 -> initial barrier state uniform
-//   role 2: catch-up uniform of the trajectory's n-th barrier hand-over (word 0)
+//   role 2: catch-up uniforms of the barrier hand-overs, four per call: the
+//           trajectory's n-th hand-over takes word n mod 4 of index n / 4
 //   role 3 + 256*array: state of barrier `id` when first touched (word 0)
 #pragma once
 #include <hip/hip_runtime.h>

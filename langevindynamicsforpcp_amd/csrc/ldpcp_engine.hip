@@ -315,11 +315,46 @@ __device__ __forceinline__ void flush_absorptions(const SetDev* __restrict__ Sg,
     __threadfence_block();
 }
 
+// The word of the trajectory's n-th catch-up uniform: word n mod 4 of the
+// role-2 call at index n / 4.  cw holds words 1-3 of the call at index
+// n_catch / 4 whenever n_catch mod 4 != 0, so one call serves four hand-overs
+// (a wavefront runs the call only when one of its lanes starts a new four).
+__device__ __forceinline__ uint32_t catch_word(const Hot& H, uint32_t tid, uint32_t& n_catch, uint32_t* cw)
+{
+    const uint32_t k = n_catch & 3u;
+    uint32_t word;
+    if (k != 0u) {
+        word = k == 1u ? cw[0] : (k == 2u ? cw[1] : cw[2]);
+    } else {
+        const Philox4 w = philox4x32_10(n_catch >> 2, 0u, tid, 2u, H.key0, H.key1);
+        cw[0] = w.w[1];
+        cw[1] = w.w[2];
+        cw[2] = w.w[3];
+        word = w.w[0];
+    }
+    ++n_catch;
+    return word;
+}
+
+// the catch-up cache at the start of a launch (catch_word)
+__device__ __forceinline__ void catch_cache_init(const Hot& H, uint32_t tid, uint32_t n_catch, uint32_t* cw)
+{
+    cw[0] = cw[1] = cw[2] = 0u;
+    if (!H.tau_zero && (n_catch & 3u)) {
+        const Philox4 w = philox4x32_10(n_catch >> 2, 0u, tid, 2u, H.key0, H.key1);
+        cw[0] = w.w[1];
+        cw[1] = w.w[2];
+        cw[2] = w.w[3];
+    }
+}
+
 // catch-up flip on entering a barrier (langevinDynamicsSingleBarrier.h:2039-2051):
 // state of the barrier entered, aged by the steps since it was last left.
+// cw: the lane's catch-up word cache (catch_word).
 template <typename Real, int MODEL>
 __device__ __forceinline__ int handover(const Hot& H, const SetDev* __restrict__ Sg, uint32_t& flags, uint32_t* ts,
-                                        int kp, int kc, int state, uint32_t step, uint32_t tid, uint32_t& n_catch)
+                                        int kp, int kc, int state, uint32_t step, uint32_t tid, uint32_t& n_catch,
+                                        uint32_t* cw)
 {
     // the single barrier uses slots 0 and 1 only (one per cell): two-way
     // selects instead of four-way ones for the lane's slot index
@@ -352,8 +387,7 @@ __device__ __forceinline__ int handover(const Hot& H, const SetDev* __restrict__
         ts_set(kc, 0u);
     }
     if (H.tau_zero) return prev;
-    const Philox4 w = philox4x32_10(n_catch, 0u, tid, 2u, H.key0, H.key1);
-    n_catch++;
+    const uint32_t word = catch_word(H, tid, n_catch, cw);
     const Real lap = (Real)(step - ts_c);
     Real pc;
     if (sizeof(Real) == 4) {
@@ -368,7 +402,7 @@ __device__ __forceinline__ int handover(const Hot& H, const SetDev* __restrict__
         pc = (Real)(scale * (1.0 - e));
     }
     // the catch-up uniform in the stepping precision
-    const bool flip = (sizeof(Real) == 4 ? (Real)unit_f32(w.w[0]) : (Real)unit_f64(w.w[0])) < pc;
+    const bool flip = (sizeof(Real) == 4 ? (Real)unit_f32(word) : (Real)unit_f64(word)) < pc;
     return flip ? (prev ^ 1) : prev;
 }
 
@@ -454,6 +488,8 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
     const uint4 t4 = P.ts[slot];
     uint32_t ts[4] = {t4.x, t4.y, t4.z, t4.w};
     uint32_t n_catch = P.n_catch[slot];
+    uint32_t cw[3];
+    catch_cache_init(H, tid, n_catch, cw);
     uint32_t tpt0 = P.tpt0[slot];
     uint32_t xf = 0, xl = 0;
     if (MODEL == 1) {
@@ -507,7 +543,7 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
             if (ent_period) {
                 const int kp = slot_index(0, cell_prev, H.slot_base, 0);
                 const int kc = slot_index(0, cell, H.slot_base, 0);
-                state = handover<float, MODEL>(H, Sg, flags, ts, kp, kc, state, step, tid, n_catch);
+                state = handover<float, MODEL>(H, Sg, flags, ts, kp, kc, state, step, tid, n_catch, cw);
             }
         } else {
             const int local_prev = local;
@@ -515,7 +551,7 @@ __device__ __forceinline__ void body_f32(const SlotPtrs& P, const StatsPtrs& St,
             if (local != local_prev) {
                 const int kp = slot_index(1, cell_prev, H.slot_base, local_prev);
                 const int kc = slot_index(1, cell, H.slot_base, local);
-                state = handover<float, MODEL>(H, Sg, flags, ts, kp, kc, state, step, tid, n_catch);
+                state = handover<float, MODEL>(H, Sg, flags, ts, kp, kc, state, step, tid, n_catch, cw);
                 if (!ent_period) {
                     xl = step;
                     if (!xf) xf = step;
@@ -646,6 +682,8 @@ __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St,
     const uint4 t4 = P.ts[slot];
     uint32_t ts[4] = {t4.x, t4.y, t4.z, t4.w};
     uint32_t n_catch = P.n_catch[slot];
+    uint32_t cw[3];
+    catch_cache_init(H, tid, n_catch, cw);
     uint32_t tpt0 = P.tpt0[slot];
     uint32_t xf = 0, xl = 0;
     if (MODEL == 1) {
@@ -686,7 +724,7 @@ __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St,
             if (ent_period) {
                 const int kp = slot_index(0, (int)periodic_prev, H.slot_base, 0);
                 const int kc = slot_index(0, (int)periodic, H.slot_base, 0);
-                state = handover<double, MODEL>(H, Sg, flags, ts, kp, kc, state, step, tid, n_catch);
+                state = handover<double, MODEL>(H, Sg, flags, ts, kp, kc, state, step, tid, n_catch, cw);
             }
         } else {
             const int local_prev = local;
@@ -694,7 +732,7 @@ __device__ __forceinline__ void body_f64(const SlotPtrs& P, const StatsPtrs& St,
             if (local != local_prev) {
                 const int kp = slot_index(1, (int)periodic_prev, H.slot_base, local_prev);
                 const int kc = slot_index(1, (int)periodic, H.slot_base, local);
-                state = handover<double, MODEL>(H, Sg, flags, ts, kp, kc, state, step, tid, n_catch);
+                state = handover<double, MODEL>(H, Sg, flags, ts, kp, kc, state, step, tid, n_catch, cw);
                 if (!ent_period) {
                     xl = step;
                     if (!xf) xf = step;

@@ -1022,9 +1022,11 @@ __device__ __forceinline__ HandOver hand_over(const RunConst& c, const RunPtrs& 
         }
         uc = P.c[(uint64_t)t * c.catch_cap + n_catch];
     } else {
-        const Philox4 u4 = philox4x32_10((uint32_t)n_catch, (uint32_t)(n_catch >> 32), c.traj_offset + t, 2u, c.key0,
-                                         c.key1);
-        uc = unit_f64(u4.w[0]);
+        // four catch-up uniforms per call: word n mod 4 of index n / 4
+        const uint64_t q = n_catch >> 2;
+        const Philox4 u4 = philox4x32_10((uint32_t)q, (uint32_t)(q >> 32), c.traj_offset + t, 2u, c.key0, c.key1);
+        const uint32_t k = (uint32_t)n_catch & 3u;
+        uc = unit_f64(k == 0 ? u4.w[0] : k == 1 ? u4.w[1] : k == 2 ? u4.w[2] : u4.w[3]);
     }
     r.n_catch = n_catch + 1;
     r.state = (uc < pc) ? ((prev_state == 1.0) ? 0.0 : 1.0) : prev_state;

@@ -484,11 +484,14 @@ static void hand_over(group* g, lane_ctx* lanes, unsigned bits, vd periodic_prev
             if (c->equal) pc = 0.5 * (1.0 - exp(c->c_lap * lap));
             else if (prev == 0.0) pc = c->K / (1.0 + c->K) * (1.0 - exp(c->c_lap * lap));
             else pc = 1.0 / (1.0 + c->K) * (1.0 - exp(c->c_lap * lap));
-            const uint32_t ctr[4] = {(uint32_t)L->n_catch, (uint32_t)(L->n_catch >> 32), L->traj, 2u};
+            /* four catch-up uniforms per call: word n mod 4 of index n / 4 */
+            const uint64_t q = L->n_catch >> 2;
+            const uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), L->traj, 2u};
             uint32_t w[4];
             philox_scalar(ctr, sw->key0, sw->key1, w);
+            const double u = unit_scalar(w[L->n_catch & 3]);
             L->n_catch++;
-            st[i] = (unit_scalar(w[0]) < pc) ? ((prev == 1.0) ? 0.0 : 1.0) : prev;
+            st[i] = (u < pc) ? ((prev == 1.0) ? 0.0 : 1.0) : prev;
         }
     }
     g->state = V_load(st);

@@ -977,10 +977,12 @@ static double noise_catch(lane_noise* n, ldo_lane* o)
         if (o->n_catch >= n->n_catch) { n->err = 1; return 1.0; }
         return n->catchup[o->n_catch++];
     }
+    /* four uniforms per call: the n-th hand-over takes word n mod 4 of index n / 4 */
     uint32_t w[4];
-    philox_role(n, o->n_catch, 2, w);
+    philox_role(n, o->n_catch >> 2, 2, w);
+    const double u = u32_to_unit(w[o->n_catch & 3]);
     o->n_catch++;
-    return u32_to_unit(w[0]);
+    return u;
 }
 
 /* state a barrier has when first touched (role 3 / replay table) */
