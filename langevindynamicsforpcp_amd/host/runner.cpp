@@ -23,6 +23,29 @@ namespace ldpcp_host {
 
 namespace {
 
+// Rank handles integrating at once (one stream each).
+constexpr int kMaxHandlesInFlight = 30;
+// Upper bound on the hardware queues the programs ask the HIP runtime for.
+constexpr int kMaxHwQueues = 32;
+
+// Each rank's handle runs its kernels on a stream of its own.  The runtime
+// gives a process GPU_MAX_HW_QUEUES hardware queues (4 by default); streams
+// beyond that share queues and their kernels run by turns, so a run of more
+// ranks than queues takes proportionally longer.  Before the first HIP call,
+// raise the variable to the handles in flight plus two for the runtime's own
+// copies (at most kMaxHwQueues), unless it is already that high or
+// LDPCP_KEEP_HW_QUEUES is set.
+void reserve_hw_queues(int handles)
+{
+    if (getenv("LDPCP_KEEP_HW_QUEUES")) return;
+    const int want = std::min(kMaxHwQueues, handles + 2);
+    const char* cur = getenv("GPU_MAX_HW_QUEUES");
+    if (cur && atoi(cur) >= want) return;
+    char buf[16];
+    snprintf(buf, sizeof(buf), "%d", want);
+    setenv("GPU_MAX_HW_QUEUES", buf, 1);
+}
+
 // HISTOGRAM::HISTOGRAM (clsHistogram.cpp:96-263): AddValue truncates
 // (v - min) / step to int; HistogramVector never counts (its loop bounds are
 // taken before `last = -1` is clamped, clsHistogram.cpp:242-254), so the
@@ -380,10 +403,11 @@ int run(const RunnerConfig& cfg)
     std::vector<char> image;
 
     // Trials are independent unless UNIFORMINITALSTATES carries each lane's
-    // initial state into the next one, so up to 64 handles (ranks x trials) are
+    // initial state into the next one, so up to 30 handles (ranks x trials) are
     // launched before any is waited for; the post analysis then runs in the
     // reference's order, trial by trial and rank by rank.
-    const int batch = cfg.same_initial_barrier ? 1 : std::max(1, 64 / R);
+    const int batch = cfg.same_initial_barrier ? 1 : std::max(1, kMaxHandlesInFlight / R);
+    reserve_hw_queues(std::min(cfg.ntrial, batch) * R);
     for (int i0 = 0; i0 < cfg.ntrial; i0 += batch) {
         const int i1 = std::min(cfg.ntrial, i0 + batch);
         const auto t0 = clock::now();
