@@ -265,33 +265,39 @@ __device__ __forceinline__ void flush_absorptions(const SetDev* __restrict__ Sg,
                                                   const Absorb a, uint64_t absorbed_mask, int lane, int leader,
                                                   uint32_t* __restrict__ hist)
 {
-    const double dt = uni(Sg->dt);
     const bool on = a.step != 0u;
-    // one sum at a time, each total moved to scalar registers, so that the
-    // reduction holds two VGPRs rather than sixteen
-    auto wave_total = [&](double x) __attribute__((always_inline)) {
+    // The durations are whole numbers of steps: their sums and sums of squares
+    // are added as integers (the squares as 128-bit words), so the totals do
+    // not depend on the order in which wavefronts add them.  One sum at a time,
+    // each total moved to scalar registers.
+    unsigned long long* sm = St.sums + (uint64_t)set * kSumsPerSet;
+    auto add = [&](int k, uint32_t n) __attribute__((always_inline)) {
+        uint64_t s1 = n;
+        uint64_t lo = (uint64_t)n * n, hi = 0;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-        return uni(x);
+        for (int off = 32; off > 0; off >>= 1) {
+            s1 += (uint64_t)__shfl_xor((unsigned long long)s1, off);
+            const uint64_t olo = (uint64_t)__shfl_xor((unsigned long long)lo, off);
+            const uint64_t ohi = (uint64_t)__shfl_xor((unsigned long long)hi, off);
+            const uint64_t sum = lo + olo;
+            hi += ohi + (sum < lo ? 1u : 0u);
+            lo = sum;
+        }
+        s1 = uni(s1);
+        lo = uni(lo);
+        hi = uni(hi);
+        if (lane == leader) {
+            atomicAdd(sm + 3 * k, (unsigned long long)s1);
+            // the high word takes the carry out of this addition to the low word
+            const uint64_t old = atomicAdd(sm + 3 * k + 1, (unsigned long long)lo);
+            atomicAdd(sm + 3 * k + 2, (unsigned long long)(hi + (old + lo < old ? 1u : 0u)));
+        }
     };
-    double* sm = St.sums + (uint64_t)set * kSumsPerSet;
-    auto add = [&](int i, double x) __attribute__((always_inline)) {
-        const double t = wave_total(x);
-        if (lane == leader) atomicAdd(sm + i, t);
-    };
-    const double fpt = on ? dt * (double)a.step : 0.0;
-    add(0, fpt);
-    add(1, fpt * fpt);
-    const double tpt = on ? dt * (double)(a.step - a.tpt0) : 0.0;
-    add(2, tpt);
-    add(3, tpt * tpt);
+    add(0, on ? a.step : 0u);
+    add(1, on ? a.step - a.tpt0 : 0u);
     if (MODEL == 1) {
-        const double b0 = (on && a.xf) ? dt * (double)(a.xf - a.tpt0) : 0.0;
-        add(4, b0);
-        add(5, b0 * b0);
-        const double b1 = on ? dt * (double)(a.step - a.xl) : 0.0;
-        add(6, b1);
-        add(7, b1 * b1);
+        add(2, (on && a.xf) ? a.xf - a.tpt0 : 0u);
+        add(3, on ? a.step - a.xl : 0u);
     }
     if (lane == leader)
         atomicAdd(St.cnt + (uint64_t)set * kCntPerSet + 0, (unsigned long long)__popcll(absorbed_mask));
@@ -1053,7 +1059,7 @@ __global__ void __launch_bounds__(256) engine_zero_set(StatsPtrs St, int set)
         St.msd_sum[(uint64_t)set * LDPCP_MSD_LAGS + t] = 0.0;
         St.msd_n[(uint64_t)set * LDPCP_MSD_LAGS + t] = 0ull;
     }
-    if (t < kSumsPerSet) St.sums[(uint64_t)set * kSumsPerSet + t] = 0.0;
+    if (t < kSumsPerSet) St.sums[(uint64_t)set * kSumsPerSet + t] = 0ull;
     if (t < kCntPerSet) St.cnt[(uint64_t)set * kCntPerSet + t] = 0ull;
 }
 
@@ -1397,7 +1403,7 @@ int Engine::init(int precision, uint64_t capacity, int max_sets)
     d_off_ = (uint32_t*)(b + o_off);
     d_sets_ = (SetDev*)(b + o_sets);
     St_.hist = (unsigned long long*)(b + o_hist);
-    St_.sums = (double*)(b + o_sums);
+    St_.sums = (unsigned long long*)(b + o_sums);
     St_.cnt = (unsigned long long*)(b + o_cnt);
     St_.msd_sum = (double*)(b + o_msd);
     St_.msd_n = (unsigned long long*)(b + o_msdn);
@@ -1622,7 +1628,7 @@ int Engine::stats(int id, ldpcp_set_stats* out, hipStream_t s)
     }
     const hipStream_t st = s ? s : stream_;
     memset(out, 0, sizeof(*out));
-    double sums[kSumsPerSet];
+    unsigned long long sums[kSumsPerSet];
     unsigned long long cnt[kCntPerSet];
     LDPCP_HIP_TRY(hipMemcpyAsync(sums, St_.sums + (uint64_t)id * kSumsPerSet, sizeof(sums), hipMemcpyDeviceToHost, st));
     LDPCP_HIP_TRY(hipMemcpyAsync(cnt, St_.cnt + (uint64_t)id * kCntPerSet, sizeof(cnt), hipMemcpyDeviceToHost, st));
@@ -1640,14 +1646,20 @@ int Engine::stats(int id, ldpcp_set_stats* out, hipStream_t s)
     }
     const SetDev& d = h_sets_[id];
     out->n_traj = h_params_[id].n_traj;
-    out->fpt_sum = sums[0];
-    out->fpt_sum2 = sums[1];
-    out->tpt_sum = sums[2];
-    out->tpt_sum2 = sums[3];
-    out->tptb_sum[0] = sums[4];
-    out->tptb_sum2[0] = sums[5];
-    out->tptb_sum[1] = sums[6];
-    out->tptb_sum2[1] = sums[7];
+    // the integer step sums in time units: dt * sum and dt^2 * sum of squares
+    const double dt = d.dt;
+    auto sum1 = [&](int k) { return dt * (double)sums[3 * k]; };
+    auto sum2 = [&](int k) {
+        return dt * dt * ((double)sums[3 * k + 2] * 18446744073709551616.0 + (double)sums[3 * k + 1]);
+    };
+    out->fpt_sum = sum1(0);
+    out->fpt_sum2 = sum2(0);
+    out->tpt_sum = sum1(1);
+    out->tpt_sum2 = sum2(1);
+    out->tptb_sum[0] = sum1(2);
+    out->tptb_sum2[0] = sum2(2);
+    out->tptb_sum[1] = sum1(3);
+    out->tptb_sum2[1] = sum2(3);
     out->n_absorbed = cnt[0];
     out->n_finished = cnt[1];
     out->n_capped = cnt[1] - cnt[0];

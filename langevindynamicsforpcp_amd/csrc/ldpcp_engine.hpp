@@ -44,7 +44,10 @@ struct SlotPtrs {
 // Device pointers of the per-set statistics.
 struct StatsPtrs {
     unsigned long long* hist;   // [set][LDPCP_HIST_BINS]
-    double* sums;               // [set][8]: fpt, fpt^2, tpt, tpt^2, tptb0, tptb0^2, tptb1, tptb1^2
+    // [set][12]: for fpt, tpt, tptb0, tptb1 (in steps) the sum, and the sum of
+    // squares as a 128-bit integer (low, high word): exact, so the same
+    // whatever order the wavefronts add in
+    unsigned long long* sums;
     unsigned long long* cnt;    // [set][4]: n_absorbed, n_finished, traj_steps, unused
     double* msd_sum;            // [set][LDPCP_MSD_LAGS]
     unsigned long long* msd_n;  // [set][LDPCP_MSD_LAGS]
@@ -60,7 +63,7 @@ struct Ctrl {
     unsigned long long launches;
 };
 
-constexpr int kSumsPerSet = 8;
+constexpr int kSumsPerSet = 12;
 constexpr int kCntPerSet = 4;
 
 class Engine {

@@ -237,3 +237,11 @@ def simd_absorb(lib, params, seeds, n_traj=0, seconds=0.0, threads=1, want_paths
     if rc != 0:
         raise RuntimeError(f"ldo_simd_absorb returned {rc}")
     return ab, tp, steps.value, done.value, wall.value
+
+
+def step_sums(dt, n):
+    """(sum, sum of squares) of durations n (in steps) as the engine reports
+    them: dt * sum(n) and dt^2 * sum(n^2), the integer sums rounded once."""
+    n = [int(x) for x in n]
+    s1, s2 = sum(n), sum(x * x for x in n)
+    return dt * float(s1), dt * dt * (float(s2 >> 64) * 18446744073709551616.0 + float(s2 & (2 ** 64 - 1)))

@@ -26,6 +26,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as ol
+from oracle_lib import step_sums
 from langevindynamicsforpcp_amd import workloads as WL
 from langevindynamicsforpcp_amd.sweep import Sweep
 
@@ -111,15 +112,13 @@ def check_against_oracle(oracle, sets):
         m = ab > 0
         absorbed_total += int(m.sum())
         assert st.n_absorbed == int(m.sum()) and st.n_absorbed + st.n_capped == s.n_traj
-        fpt = ab[m].astype(float) * s.timestep
-        assert st.fpt_sum == pytest.approx(fpt.sum(), rel=1e-12, abs=1e-300)
-        assert st.tpt_sum == pytest.approx(((ab[m] - tp[m]).astype(float) * s.timestep).sum(), rel=1e-12,
-                                           abs=1e-300)
+        # the oracle's steps give the engine's duration sums exactly (integer step sums)
+        assert (st.fpt_sum, st.fpt_sum2) == step_sums(s.timestep, ab[m])
+        assert (st.tpt_sum, st.tpt_sum2) == step_sums(s.timestep, ab[m] - tp[m])
         if s.model == 1:
-            b0 = np.where(xf[m] > 0, (xf[m] - tp[m]).astype(float) * s.timestep, 0.0).sum()
-            b1 = ((ab[m] - xl[m]).astype(float) * s.timestep).sum()
-            assert st.tptb_sum[0] == pytest.approx(b0, rel=1e-12, abs=1e-300)
-            assert st.tptb_sum[1] == pytest.approx(b1, rel=1e-12, abs=1e-300)
+            b0 = np.where(xf[m] > 0, xf[m] - tp[m], 0)
+            assert (st.tptb_sum[0], st.tptb_sum2[0]) == step_sums(s.timestep, b0)
+            assert (st.tptb_sum[1], st.tptb_sum2[1]) == step_sums(s.timestep, ab[m] - xl[m])
         assert list(np.array(st.msd_n)[use]) == list(want_n), i
         np.testing.assert_allclose(np.array(st.msd_sum)[use], want, rtol=1e-12)
         # histogram: every bin equal to the FP64 log10 binning of the oracle's steps
@@ -285,6 +284,6 @@ def test_serving_slice_equals_runs_alone():
             assert getattr(got, f) == getattr(alone, f), (k, f)
         assert list(got.hist) == list(alone.hist), k
         assert list(got.msd_n) == list(alone.msd_n), k
-        assert got.fpt_sum == pytest.approx(alone.fpt_sum, rel=1e-12, abs=1e-300)
-        assert got.tpt_sum == pytest.approx(alone.tpt_sum, rel=1e-12, abs=1e-300)
+        assert (got.fpt_sum, got.fpt_sum2, got.tpt_sum, got.tpt_sum2) == \
+            (alone.fpt_sum, alone.fpt_sum2, alone.tpt_sum, alone.tpt_sum2), k
         np.testing.assert_allclose(np.array(got.msd_sum), np.array(alone.msd_sum), rtol=1e-9)

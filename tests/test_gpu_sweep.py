@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as ol
+from oracle_lib import step_sums
 from langevindynamicsforpcp_amd.sweep import Sweep, sweep_set
 
 pytestmark = pytest.mark.gpu
@@ -34,6 +35,7 @@ def oracle_absorb(oracle, s, traj, absorb):
     out = ol.Lane()
     oracle.ldo_lane_run_philox(C.byref(p), C.byref(out), s.seed, traj, 0, 0, None, 0)
     return out
+
 
 
 @pytest.mark.parametrize("model,over", [(0, True), (0, False), (1, True), (1, False)])
@@ -67,16 +69,14 @@ def test_fp64_sweep_matches_oracle(oracle, model, over):
     assert sum(st.hist) == st.n_absorbed
     assert st.n_absorbed > s.n_traj // 2
     m = ab > 0
-    fpt = ab[m].astype(float) * s.timestep
-    assert st.fpt_sum == pytest.approx(fpt.sum(), rel=1e-12)
-    assert st.tpt_sum == pytest.approx(((ab[m] - tp[m]).astype(float) * s.timestep).sum(), rel=1e-12)
+    # the oracle's steps give the engine's sums exactly (integer step sums)
+    assert (st.fpt_sum, st.fpt_sum2) == step_sums(s.timestep, ab[m])
+    assert (st.tpt_sum, st.tpt_sum2) == step_sums(s.timestep, ab[m] - tp[m])
     if model == 1:
         # per-barrier transition paths: well centre -> xc, last xc crossing -> target
         assert np.all(xf[m] > tp[m]) and np.all(xl[m] >= xf[m]) and np.all(xl[m] < ab[m])
-        b0 = ((xf[m] - tp[m]).astype(float) * s.timestep).sum()
-        b1 = ((ab[m] - xl[m]).astype(float) * s.timestep).sum()
-        assert st.tptb_sum[0] == pytest.approx(b0, rel=1e-12)
-        assert st.tptb_sum[1] == pytest.approx(b1, rel=1e-12)
+        assert (st.tptb_sum[0], st.tptb_sum2[0]) == step_sums(s.timestep, xf[m] - tp[m])
+        assert (st.tptb_sum[1], st.tptb_sum2[1]) == step_sums(s.timestep, ab[m] - xl[m])
     sw.close()
 
 
