@@ -280,7 +280,10 @@ typedef struct ldpcp_sweep_set {
 /* Statistics of one parameter set.  The first passage event is the first
  * barrier-top crossing for the single barrier (the reference's first FPT1
  * sample) and the first arrival at a non-adjacent well centre for the double
- * barrier (its first FPT2 sample).  Times are in the set's time units. */
+ * barrier (its first FPT2 sample).  Times are in the set's time units.  The
+ * duration sums are kept on the device as exact integer step sums and read
+ * as dt * sum and dt^2 * sum of squares: bit-identical whatever the launch
+ * shape or the other sets resident. */
 typedef struct ldpcp_set_stats {
     uint64_t n_traj, n_absorbed, n_capped;
     double fpt_sum, fpt_sum2;      /* first-passage times of absorbed trajectories */

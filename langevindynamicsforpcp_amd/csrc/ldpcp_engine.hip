@@ -255,7 +255,7 @@ struct Absorb {
 constexpr size_t kLdsPerWave = LDPCP_MSD_LAGS * 8 + LDPCP_MSD_LAGS * 4 + LDPCP_HIST_BINS * 4;
 
 // The first passages of one wavefront's lanes of one set (lanes of other sets
-// and lanes without one pass step 0): the eight FPT/TPT sums are reduced
+// and lanes without one pass step 0): the FPT/TPT step sums are reduced
 // across the wavefront and added once by the set's leading lane; each passage
 // is counted in the wavefront's LDS histogram, whose non-zero bins are then
 // added to the set's histogram.  The bin of absorption step s is the last k
