@@ -358,6 +358,7 @@ typedef struct ldpcp_server_config {
     int32_t precision;             /* 32 or 64 */
     int32_t max_requests;          /* resident requests at once (0 = 4096) */
     ldpcp_engine_config engine;
+    int32_t keep_fpt;              /* 1: keep each completed request's per-trajectory lists (ldpcp_server_fpt) */
 } ldpcp_server_config;
 
 enum { LDPCP_REQ_QUEUED = 0, LDPCP_REQ_RUNNING = 1, LDPCP_REQ_DONE = 2 };
@@ -373,6 +374,12 @@ int ldpcp_server_pump(ldpcp_server* s, uint32_t n_launches, int32_t wait);
 int ldpcp_server_status(ldpcp_server* s, uint64_t request_id, int32_t* state);
 /* Ids of requests completed since the last call (at most max). */
 int ldpcp_server_completed(ldpcp_server* s, uint64_t* ids, int32_t max, int32_t* n);
+/* Per trajectory of a completed request, in the layout of ldpcp_sweep_fpt
+ * (n_traj entries each; the values equal those of the same set run alone by
+ * a sweep).  Needs keep_fpt; call before ldpcp_server_result, which frees the
+ * record with the lists.  xc_first / xc_last are zero for the single barrier. */
+int ldpcp_server_fpt(ldpcp_server* s, uint64_t request_id, uint64_t* absorbed_step, uint64_t* tpt_start,
+                     uint64_t* xc_first, uint64_t* xc_last);
 /* Results of a completed request; frees its record. */
 int ldpcp_server_result(ldpcp_server* s, uint64_t request_id, ldpcp_set_stats* out);
 /* queued / resident requests, live trajectories, resident bytes, trajectory-steps done */

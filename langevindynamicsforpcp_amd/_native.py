@@ -108,7 +108,7 @@ class EngineConfig(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("budget_bytes", C.c_uint64), ("precision", C.c_int32), ("max_requests", C.c_int32),
-                ("engine", EngineConfig)]
+                ("engine", EngineConfig), ("keep_fpt", C.c_int32)]
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
@@ -165,6 +165,7 @@ _SIGNATURES = [
     ("ldpcp_server_pump", C.c_int, [_P, C.c_uint32, C.c_int32]),
     ("ldpcp_server_status", C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int32)]),
     ("ldpcp_server_completed", C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int32, C.POINTER(C.c_int32)]),
+    ("ldpcp_server_fpt", C.c_int, [_P, C.c_uint64] + [C.POINTER(C.c_uint64)] * 4),
     ("ldpcp_server_result", C.c_int, [_P, C.c_uint64, C.POINTER(SetStats)]),
     ("ldpcp_server_info", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),

@@ -1672,7 +1672,8 @@ int Engine::stats(int id, ldpcp_set_stats* out, hipStream_t s)
     return LDPCP_OK;
 }
 
-int Engine::per_traj(int id, uint64_t* absorbed, uint64_t* tpt0, uint64_t* xc_first, uint64_t* xc_last)
+int Engine::per_traj(int id, uint64_t* absorbed, uint64_t* tpt0, uint64_t* xc_first, uint64_t* xc_last,
+                     hipStream_t s)
 {
     if (id < 0 || id >= max_sets_ || h_params_[id].n_traj == 0 || !absorbed) {
         set_error("engine: no resident set %d or no output", id);
@@ -1684,9 +1685,15 @@ int Engine::per_traj(int id, uint64_t* absorbed, uint64_t* tpt0, uint64_t* xc_fi
     uint64_t* dst[4] = {absorbed, tpt0, xc_first, xc_last};
     for (int k = 0; k < 4; ++k) {
         if (!dst[k]) continue;
-        LDPCP_HIP_TRY(hipMemcpyAsync(buf.data(), src[k] + first, n * 4, hipMemcpyDeviceToHost, stream_));
-        const int rc = sync();
-        if (rc) return rc;
+        // as in stats(): another stream reads a set whose trajectories have
+        // all finished without waiting for the launches in flight
+        LDPCP_HIP_TRY(hipMemcpyAsync(buf.data(), src[k] + first, n * 4, hipMemcpyDeviceToHost, s ? s : stream_));
+        if (s) {
+            LDPCP_HIP_TRY(hipStreamSynchronize(s));
+        } else {
+            const int rc = sync();
+            if (rc) return rc;
+        }
         for (uint64_t i = 0; i < n; ++i) dst[k][i] = buf[i];
     }
     return LDPCP_OK;

@@ -90,7 +90,8 @@ public:
     // blocking reads; stats() of a set whose trajectories have all finished
     // may use another stream, so that it does not wait for launches in flight
     int stats(int set_id, ldpcp_set_stats* out, hipStream_t s = nullptr);
-    int per_traj(int set_id, uint64_t* absorbed, uint64_t* tpt0, uint64_t* xc_first, uint64_t* xc_last);
+    int per_traj(int set_id, uint64_t* absorbed, uint64_t* tpt0, uint64_t* xc_first, uint64_t* xc_last,
+                 hipStream_t s = nullptr);
     int live_count(uint64_t* live);
     // frees a set's slots and id (its trajectories must all have finished)
     void release(int set_id);

@@ -7,7 +7,8 @@
 // run in the same persistent launches as everything already resident.  After
 // every launch the per-set finished counters are copied to pinned host memory;
 // the host reads the copy of launch i while launch i+1 runs, and a request
-// whose last trajectory has finished has its statistics read on a second
+// whose last trajectory has finished has its statistics (and, when the
+// server keeps them, its per-trajectory first-passage lists) read on a second
 // stream, its slots freed and its id reported as completed.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -30,6 +31,9 @@ struct Request {
     int state = LDPCP_REQ_QUEUED;
     int engine_id = -1;
     ldpcp_set_stats result;
+    // per trajectory, when the server keeps them: absorption step (0 = capped),
+    // TPT start, first and last xc crossing (double barrier)
+    std::vector<uint64_t> fpt[4];
 };
 
 }  // namespace
@@ -37,6 +41,7 @@ struct Request {
 struct ldpcp_server {
     Engine eng;
     uint64_t budget = 0;
+    bool keep_fpt = false;
     uint64_t resident_bytes = 0;
     std::deque<uint64_t> queue;
     std::unordered_map<uint64_t, Request> reqs;
@@ -106,6 +111,14 @@ struct ldpcp_server {
             // the set's statistics are final
             rc = eng.stats(id, &r.result, copy_stream);
             if (rc) return rc;
+            if (keep_fpt) {
+                const size_t n = (size_t)r.set.n_traj;
+                for (auto& v : r.fpt) v.assign(n, 0);
+                const bool dbl = r.set.model == 1;
+                rc = eng.per_traj(id, r.fpt[0].data(), r.fpt[1].data(), dbl ? r.fpt[2].data() : nullptr,
+                                  dbl ? r.fpt[3].data() : nullptr, copy_stream);
+                if (rc) return rc;
+            }
             eng.release(id);
             by_engine_id[id] = 0;
             resident_bytes -= r.bytes;
@@ -121,8 +134,9 @@ struct ldpcp_server {
 
 extern "C" int ldpcp_server_create(const ldpcp_server_config* cfg, ldpcp_server** out)
 {
-    if (!cfg || !out || (cfg->precision != 32 && cfg->precision != 64) || cfg->max_requests < 0) {
-        set_error("ldpcp_server_create: need a config with precision 32 or 64 and max_requests >= 0");
+    if (!cfg || !out || (cfg->precision != 32 && cfg->precision != 64) || cfg->max_requests < 0 ||
+        (cfg->keep_fpt != 0 && cfg->keep_fpt != 1)) {
+        set_error("ldpcp_server_create: need a config with precision 32 or 64, max_requests >= 0 and keep_fpt 0 or 1");
         return LDPCP_EINVAL;
     }
     *out = nullptr;
@@ -151,6 +165,7 @@ extern "C" int ldpcp_server_create(const ldpcp_server_config* cfg, ldpcp_server*
         return rc;
     }
     s->budget = cfg->budget_bytes;
+    s->keep_fpt = cfg->keep_fpt == 1;
     s->by_engine_id.assign((size_t)max_sets, 0);
     s->steps_of.assign((size_t)max_sets, 0);
     s->admitted_at.assign((size_t)max_sets, 0);
@@ -277,6 +292,36 @@ extern "C" int ldpcp_server_result(ldpcp_server* s, uint64_t request_id, ldpcp_s
     }
     *out = it->second.result;
     s->reqs.erase(it);
+    return LDPCP_OK;
+}
+
+extern "C" int ldpcp_server_fpt(ldpcp_server* s, uint64_t request_id, uint64_t* absorbed_step, uint64_t* tpt_start,
+                                uint64_t* xc_first, uint64_t* xc_last)
+{
+    if (!s || !absorbed_step) {
+        set_error("ldpcp_server_fpt: null argument");
+        return LDPCP_EINVAL;
+    }
+    if (!s->keep_fpt) {
+        set_error("ldpcp_server_fpt: the server was created without keep_fpt");
+        return LDPCP_EINVAL;
+    }
+    auto it = s->reqs.find(request_id);
+    if (it == s->reqs.end() || it->second.state != LDPCP_REQ_DONE) {
+        set_error("ldpcp_server_fpt: request %llu is unknown, not done or its result was read",
+                  (unsigned long long)request_id);
+        return LDPCP_EINVAL;
+    }
+    const Request& r = it->second;
+    uint64_t* dst[4] = {absorbed_step, tpt_start, xc_first, xc_last};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k]) {
+            if (r.fpt[k].empty()) {
+                memset(dst[k], 0, (size_t)r.set.n_traj * 8);
+            } else {
+                memcpy(dst[k], r.fpt[k].data(), r.fpt[k].size() * 8);
+            }
+        }
     return LDPCP_OK;
 }
 

@@ -21,12 +21,13 @@ from .sweep import Sweep, sweep_set
 
 class Server:
     def __init__(self, budget_bytes, precision=32, max_requests=0, block=0, blocks_per_cu=0, steps_per_launch=0,
-                 use_graph=0):
+                 use_graph=0, keep_fpt=False):
         cfg = N.ServerConfig()
         cfg.budget_bytes = int(budget_bytes)
         cfg.precision = int(precision)
         cfg.max_requests = int(max_requests)
         cfg.engine = N.EngineConfig(block, blocks_per_cu, steps_per_launch, use_graph)
+        cfg.keep_fpt = 1 if keep_fpt else 0
         h = C.c_void_p()
         N.check(N.load().ldpcp_server_create(C.byref(cfg), C.byref(h)), "ldpcp_server_create")
         self._h = h
@@ -58,6 +59,16 @@ class Server:
             out += list(buf[:n.value])
             if n.value < 256:
                 return out
+
+    def paths(self, rid, n_traj):
+        """Per trajectory of a completed request (server made with keep_fpt):
+        absorption step (0 = capped), last well-centre crossing before it,
+        first / last crossing of the intermediate-well bottom (double barrier).
+        Read before result(), which frees the request's record."""
+        out = [np.zeros(int(n_traj), np.uint64) for _ in range(4)]
+        ptr = [a.ctypes.data_as(C.POINTER(C.c_uint64)) for a in out]
+        N.check(N.load().ldpcp_server_fpt(self._h, int(rid), *ptr), "ldpcp_server_fpt")
+        return out
 
     def result(self, rid):
         st = N.SetStats()

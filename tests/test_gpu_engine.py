@@ -132,19 +132,25 @@ def test_sets_alone_and_together_agree():
 def test_server_results_same_alone_and_in_mix():
     sets = mixed_sets(seed0=120, cap=40000)
     print("[gpu engine] server mix", flush=True)
-    srv = Server(budget_bytes=256 << 20, precision=32, steps_per_launch=1500)
+    srv = Server(budget_bytes=256 << 20, precision=32, steps_per_launch=1500, keep_fpt=True)
     ids = []
     # requests join at different launches while others are resident
     for k, s in enumerate(sets):
         ids.append(srv.submit(s))
         srv.pump(k + 1)
-    done = {}
+    n_of = dict(zip(ids, (s.n_traj for s in sets)))
+    done, paths = {}, {}
     while len(done) < len(ids):
         srv.pump(4, wait=True)
         for rid in srv.completed():
+            paths[rid] = srv.paths(rid, n_of[rid])
             done[rid] = srv.result(rid)
     for s, rid in zip(sets, ids):
-        (_, st_alone), = run_sweep([s], 32)
+        (p_alone, st_alone), = run_sweep([s], 32)
+        # per-trajectory lists: absorption step, TPT start, xc crossings
+        for x, y in zip(paths[rid], p_alone):
+            assert np.array_equal(x, y)
+        assert np.count_nonzero(paths[rid][0]) == st_alone.n_absorbed
         st = done[rid]
         for f in ("n_traj", "n_absorbed", "n_capped", "n_finished", "traj_steps"):
             assert getattr(st, f) == getattr(st_alone, f), f
@@ -155,6 +161,9 @@ def test_server_results_same_alone_and_in_mix():
         np.testing.assert_allclose(np.array(st.msd_sum), np.array(st_alone.msd_sum), rtol=1e-9)
     info = srv.info()
     assert info["resident"] == 0 and info["queued"] == 0
+    # the lists go with the record: gone once the result is read
+    with pytest.raises(N.LdpcpError):
+        srv.paths(ids[0], n_of[ids[0]])
     srv.close()
 
 
@@ -177,6 +186,8 @@ def test_server_budget_bounds_resident_state():
         assert info["resident"] <= 2
         seen_queue |= info["queued"] > 0
         for rid in srv.completed():
+            with pytest.raises(N.LdpcpError):  # the server keeps no per-trajectory lists
+                srv.paths(rid, 20000)
             done[rid] = srv.result(rid)
     assert seen_queue
     for s, rid in zip(sets, ids):
