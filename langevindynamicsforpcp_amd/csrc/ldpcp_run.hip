@@ -1064,8 +1064,9 @@ __device__ __forceinline__ int stats_event_at(const RunConst& c, const RunPtrs& 
 // stepping wavefront's dependent chain holds the dynamics only (the noise is
 // computed in parallel over all trajectory-steps beforehand).
 constexpr int kNoisePhilox = 0, kNoiseReplay = 1, kNoiseBuffered = 2;
-constexpr int kStage = 32;  // steps per LDS stage of the buffered noise
-constexpr int kStageRows = kStage + 2;  // rows of a stage array: two rows of read-ahead padding
+// records per LDS stage array of the buffered noise: 32 steps of 64
+// trajectories and two rows of read-ahead padding
+constexpr uint32_t kStageSlots = 34u * 64u;
 // s_waitcnt immediate (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4,
 // lgkmcnt 11:8) that waits for vmcnt = 0 and lgkmcnt = 0 and leaves expcnt free
 constexpr int kWaitVmLgkm0 = 0x0070;
@@ -1084,14 +1085,21 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
 {
     constexpr int REPLAY = (SRC == kNoiseReplay);
     constexpr Flags f{MODEL, OVER, TAU0, REPLAY};
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= c.n_traj) return;
-    ldpcp_lane* const og = P.lanes + t;
-    if (og->error) return;
     // the buffered launch (one wavefront per workgroup, few trajectories) keeps the
     // statistics record and the barrier table in LDS: an event's reads and a
     // hand-over's table lookups are on the step's critical path
     constexpr bool kLocal = (SRC == kNoiseBuffered);
+    // A buffered launch keeps every lane of its wavefront: the stage loads copy
+    // 64 noise records per instruction only with all 64 lanes on.  A lane
+    // without a trajectory (beyond n_traj, or one whose record holds an error)
+    // reads an existing trajectory's data, steps as a dead lane (nothing it
+    // computes leaves its registers) and stores nothing.
+    const uint32_t t_raw = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!kLocal && t_raw >= c.n_traj) return;
+    const uint32_t t = t_raw < c.n_traj ? t_raw : c.n_traj - 1;
+    ldpcp_lane* const og = P.lanes + t;
+    const bool active = t_raw < c.n_traj && og->error == 0;
+    if (!kLocal && !active) return;
     __shared__ ldpcp_lane lane_stats[64];
     __shared__ uint64_t win_cache[2 * kWinSlots * 64];
     __shared__ int win_cache_id[2 * kWinSlots * 64];
@@ -1130,8 +1138,10 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     // The step loop runs all of the launch's steps for every trajectory (uniform
     // control flow).  A trajectory whose launch ends with an error stores its
     // record and state at that point and is dead: its later steps have no effect
-    // outside registers.
-    int dead = 0;
+    // outside registers.  live_m (wave-uniform) holds the lanes not dead, and
+    // only their crossings take the rare branch.
+    int dead = active ? 0 : 1;
+    uint64_t live_m = __ballot(active);
     // stores the state after step `last`, with `counted` steps of the launch in the state counts
     auto finish = [&](uint64_t last, uint64_t counted) __attribute__((always_inline)) {
         s.state = st ? 1.0 : 0.0;
@@ -1190,8 +1200,9 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         double F = lane_force(s, c, sp, f, st != 0);
         c1 += (uint32_t)st;
         const uint64_t m_period = lanes_ne(s.periodic, s.periodic_prev);
-        const uint64_t m_rare = m_period | lanes_ne(s.well, s.well_prev) |
-                                (f.model == 0 ? 0ull : lanes_ne(s.local, s.local_prev));
+        const uint64_t m_rare = (m_period | lanes_ne(s.well, s.well_prev) |
+                                 (f.model == 0 ? 0ull : lanes_ne(s.local, s.local_prev))) &
+                                live_m;
         if (__builtin_expect(m_rare != 0, 0)) {
             const uint64_t iStep = istep();
             const bool live = (dead == 0);
@@ -1216,6 +1227,7 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
                                      s.ent_period);
                 if (err) fail(iStep, iStep - s0);
             }
+            live_m = __ballot(dead == 0);
         }
         s.F = F;
         if (!f.over) s.v = s.v + (c.minv * s.F) * c.halfdt;
@@ -1233,47 +1245,77 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     };
 
     // the buffered noise stages (declared once: `run` below has two instantiations)
-    __shared__ uint4 stage_a[kStageRows][64], stage_b[kStageRows][64];
+    __shared__ uint4 stage_a[kStageSlots], stage_b[kStageSlots];
     auto run = [&](auto eq) __attribute__((always_inline)) {
         const uint64_t len = s1 - s0;
         if constexpr (SRC == kNoiseBuffered) {
-            // Stage q of the launch (steps s0 + 1 + 32 q ...) sits in stage_a (q even) or
-            // stage_b (q odd), loaded by direct global->LDS loads while the stage before
-            // it runs.  The even and odd stages are separate code over separate arrays,
-            // so the compiler sees that a stage's LDS reads never touch the array the
-            // next stage's loads are landing in and waits for those loads only at the
-            // stage boundary.  The launch is one wavefront per workgroup (64 threads),
-            // so a boundary's __syncthreads waits for this wavefront alone.
+            // Stage q of the launch (steps s0 + 1 + q L ..., L steps a stage) sits in
+            // stage_a (q even) or stage_b (q odd), loaded by direct global->LDS loads
+            // while the stage before it runs.  The even and odd stages are separate
+            // code over separate arrays, so the compiler sees that a stage's LDS reads
+            // never touch the array the next stage's loads are landing in and waits
+            // for those loads only at the stage boundary.  The launch is one wavefront
+            // per workgroup (64 threads), so a boundary's __syncthreads waits for this
+            // wavefront alone.
+            //
+            // A stage holds rows of S records (one per trajectory of the wavefront),
+            // row j at records j S .. j S + S - 1.  A handle of at most 64
+            // trajectories is one wavefront (S = n_traj): its rows are contiguous in
+            // the buffer ([step][trajectory]), so a stage is one block that each load
+            // copies 64 records of (one load per 64 / S steps), and a stage holds as
+            // many steps as the array does.  A wavefront of a larger handle (S = 64)
+            // loads row by row, 32 steps a stage.
             const uint32_t lane = threadIdx.x;
-            auto load_stage = [&](uint4 (&buf)[kStageRows][64], uint64_t q) {
-                const uint64_t first = q * kStage;
-                const uint64_t m = first >= len ? 0 : (len - first < kStage ? len - first : kStage);
-                for (uint64_t j = 0; j < m; ++j)
-                    __builtin_amdgcn_global_load_lds((const void*)(noise + (first + j) * c.n_traj + t),
-                                                     (__attribute__((address_space(3))) void*)&buf[j][0], 16, 0, 0);
+            const bool one_wave = c.n_traj <= 64u;
+            const uint32_t S = one_wave ? c.n_traj : 64u;
+            // steps per stage: two rows of read-ahead padding left, an even count
+            const uint32_t L = (kStageSlots / S - 2u) & ~1u;
+            // the lane's record in a row: lanes beyond the wavefront's trajectories
+            // (dead) read the first trajectory's
+            const uint32_t slot = lane < S ? lane : 0u;
+            const uint4* const wave_noise = noise + (uint64_t)blockIdx.x * 64u;
+            auto load_stage = [&](uint4 (&buf)[kStageSlots], uint64_t q) {
+                const uint64_t first = q * L;
+                const uint64_t m = first >= len ? 0 : (len - first < L ? len - first : L);
+                if (one_wave) {
+                    const uint32_t count = (uint32_t)m * S;  // records of the stage, contiguous
+                    const uint4* src = noise + first * S;
+                    for (uint32_t k0 = 0; k0 < count; k0 += 64u) {
+                        const uint32_t k = k0 + lane;
+                        __builtin_amdgcn_global_load_lds((const void*)(src + (k < count ? k : 0u)),
+                                                         (__attribute__((address_space(3))) void*)&buf[k0], 16, 0,
+                                                         0);
+                    }
+                } else {
+                    const uint32_t tl = t_raw < c.n_traj ? lane : 0u;  // in-bounds address for a dead lane
+                    for (uint64_t j = 0; j < m; ++j)
+                        __builtin_amdgcn_global_load_lds((const void*)(wave_noise + (first + j) * c.n_traj + tl),
+                                                         (__attribute__((address_space(3))) void*)&buf[j * 64u], 16,
+                                                         0, 0);
+                }
             };
             // steps of stage q from buf, two per iteration with two record registers:
             // each step's record is read during the step before (the LDS latency overlaps
             // a step), and the values a step carries to the next alternate registers
-            // instead of being copied.  A stage array has two rows beyond its kStage
+            // instead of being copied.  A stage array has two rows beyond the stage's
             // steps, so the reads one and two rows ahead stay inside it; those rows
             // are never used.
-            auto stage_steps = [&](uint4 (&buf)[kStageRows][64], uint64_t base, uint32_t m, auto frames)
+            auto stage_steps = [&](uint4 (&buf)[kStageSlots], uint64_t base, uint32_t m, auto frames)
                                    __attribute__((always_inline)) {
-                const uint4* rec = &buf[0][lane];
+                const uint4* rec = &buf[slot];
                 uint4 ra = rec[0];
                 const uint32_t pairs = m >> 1;
                 for (uint32_t k = 0; k < pairs; ++k) {
                     // rows 2k + 1 and 2k + 2 (the last read of an even stage is padding row m)
-                    const uint4 rb = rec[64];
+                    const uint4 rb = rec[S];
                     double g;
                     uint32_t fw;
                     buffered_noise(ra, g, fw);
                     step([&] { return base + 2 * k + 1; }, g, fw, eq, frames);
-                    ra = rec[128];
+                    ra = rec[2 * S];
                     buffered_noise(rb, g, fw);
                     step([&] { return base + 2 * k + 2; }, g, fw, eq, frames);
-                    rec += 128;
+                    rec += 2 * S;
                 }
                 if (m & 1) {
                     double g;
@@ -1283,9 +1325,9 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
                 }
             };
             // a stage holding a frame step runs with the frame test, the others without
-            auto run_stage = [&](uint4 (&buf)[kStageRows][64], uint64_t q) __attribute__((always_inline)) {
-                const uint64_t first = q * kStage;
-                const uint32_t m = (uint32_t)(len - first < kStage ? len - first : kStage);
+            auto run_stage = [&](uint4 (&buf)[kStageSlots], uint64_t q) __attribute__((always_inline)) {
+                const uint64_t first = q * L;
+                const uint32_t m = (uint32_t)(len - first < L ? len - first : L);
                 const uint64_t base = uniform_u64(s0 + first);
                 if (next_save <= base + m) stage_steps(buf, base, m, std::true_type{});
                 else stage_steps(buf, base, m, std::false_type{});
@@ -1297,11 +1339,11 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
                 __syncthreads();
             };
             load_stage(stage_a, 0);
-            for (uint64_t q = 0; q * kStage < len; q += 2) {
+            for (uint64_t q = 0; q * L < len; q += 2) {
                 boundary();
                 load_stage(stage_b, q + 1);
                 run_stage(stage_a, q);
-                if ((q + 1) * kStage >= len) break;
+                if ((q + 1) * L >= len) break;
                 boundary();
                 load_stage(stage_a, q + 2);
                 run_stage(stage_b, q + 1);
@@ -1342,8 +1384,10 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     else run(std::false_type{});
     if (dead == 0) finish(s1, s1 - s0);
     if constexpr (kLocal) {
-        *og = *o;
-        win_flush(w);
+        if (active) {
+            *og = *o;
+            win_flush(w);
+        }
     }
 }
 
