@@ -212,6 +212,7 @@ struct Lane {
 // asm statement, so it stays an opaque register value loaded once per launch.
 struct StateParams {
     double H[2], S[2], Sn[2];                   // single barrier [state]; Sn = 0 - S
+    uint32_t Sm[2];                             // single barrier: the sign bit that turns S into Sn (0 for S = +0)
     double dh[2][2], ds[2][2][2], di[2][2][2];  // double barrier [state][barrier][side]
     double dfn[2][2][2];                        // double barrier force 0 - ds
     double p_flip[2];                           // per-step flip probability leaving state [state]
@@ -243,6 +244,12 @@ __device__ __forceinline__ uint64_t in_register(uint64_t v)
     return v;
 }
 
+__device__ __forceinline__ uint32_t in_register(uint32_t v)
+{
+    asm("" : "+v"(v));
+    return v;
+}
+
 __device__ __forceinline__ StateParams state_params(const RunConst& c, const Flags& f)
 {
     StateParams sp;
@@ -257,6 +264,8 @@ __device__ __forceinline__ StateParams state_params(const RunConst& c, const Fla
             sp.H[st] = in_register(c.H[st]);
             sp.S[st] = in_register(c.S[st]);
             sp.Sn[st] = in_register(0.0 - c.S[st]);
+            // 0 - S is S with the sign bit flipped, except for S = +0 (0 - 0 = +0)
+            sp.Sm[st] = in_register(__double_as_longlong(c.S[st]) == 0 ? 0u : 0x80000000u);
             continue;
         }
 #pragma unroll
@@ -365,6 +374,18 @@ __device__ __forceinline__ double force_single(double x, const RunConst& c, doub
     const double Sp = st ? S1 : S0;
     const double Sm = st ? Sn1 : Sn0;
     return band ? ((xp < 0.0) ? Sm : Sp) : 0.0;
+}
+
+// force_single for the offset xp = x - (periodic index) L of the state's slope
+// S: S on (0, w), 0 - S on (-w, 0), 0 elsewhere; 0 - S is S with xp's sign bit
+// xor-ed into its high word where Sm is the sign bit.
+__device__ __forceinline__ double force_single_xp(double xp, double w, double S, uint32_t Sm)
+{
+    const bool band = (fabs(xp) < w) && (xp != 0.0);
+    const uint64_t sb = (uint64_t)__double_as_longlong(S);
+    const uint32_t hi = (uint32_t)(sb >> 32) ^ ((uint32_t)((uint64_t)__double_as_longlong(xp) >> 32) & Sm);
+    const double Fs = __hiloint2double((int)hi, (int)(uint32_t)sb);
+    return band ? Fs : 0.0;
 }
 
 // The force of EnergyAndForce for barrier state st, selected from that state's
@@ -1142,6 +1163,8 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     // only their crossings take the rare branch.
     int dead = active ? 0 : 1;
     uint64_t live_m = __ballot(active);
+    // single barrier: (periodic index) x L of the cell the coming step starts in
+    double pL = s.periodic * c.L;
     // stores the state after step `last`, with `counted` steps of the launch in the state counts
     auto finish = [&](uint64_t last, uint64_t counted) __attribute__((always_inline)) {
         s.state = st ? 1.0 : 0.0;
@@ -1197,7 +1220,11 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
             }
             st = flip ? st ^ 1 : st;
         }
-        double F = lane_force(s, c, sp, f, st != 0);
+        // the single barrier's offset from the origin of the cell the step started
+        // in (pL): exact while the periodic index is unchanged, and a step that
+        // changes it takes the rare branch, which recomputes F from x
+        double F = f.model == 0 ? force_single_xp(s.x - pL, c.w, st ? sp.S[1] : sp.S[0], st ? sp.Sm[1] : sp.Sm[0])
+                                : lane_force(s, c, sp, f, st != 0);
         c1 += (uint32_t)st;
         const uint64_t m_period = lanes_ne(s.periodic, s.periodic_prev);
         const uint64_t m_rare = (m_period | lanes_ne(s.well, s.well_prev) |
@@ -1228,6 +1255,7 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
                 if (err) fail(iStep, iStep - s0);
             }
             live_m = __ballot(dead == 0);
+            if (f.model == 0) pL = s.periodic * c.L;
         }
         s.F = F;
         if (!f.over) s.v = s.v + (c.minv * s.F) * c.halfdt;
