@@ -231,6 +231,9 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v)
 // The lanes where a != b: the compare's lane mask itself (one v_cmp; a ballot of
 // a bool would first spell the bool out in a vector register).
 __device__ __forceinline__ uint64_t lanes_ne(double a, double b) { return __builtin_amdgcn_fcmp(a, b, 14); }
+// the lanes where a < b, and where a > b, or either is NaN
+__device__ __forceinline__ uint64_t lanes_ult(double a, double b) { return __builtin_amdgcn_fcmp(a, b, 12); }
+__device__ __forceinline__ uint64_t lanes_ugt(double a, double b) { return __builtin_amdgcn_fcmp(a, b, 10); }
 
 __device__ __forceinline__ double in_register(double v)
 {
@@ -408,6 +411,21 @@ __device__ __forceinline__ double lane_force(const Lane& s, const RunConst& c, c
     F = n1 ? (st ? sp.dfn[1][1][0] : sp.dfn[0][1][0]) : F;
     F = p1 ? (st ? sp.dfn[1][1][1] : sp.dfn[0][1][1]) : F;
     return F;
+}
+
+// Positions certainly inside the single barrier's cell of (periodic, well),
+// where rint(x Linv) == periodic and floor(x Linv) == well (lane_indices):
+// the cell's ends (periodic -+ 1/2) L, well L and (well + 1) L moved inwards by
+// 2^-48 of their magnitude (at least 16 ulps; the rounding of x Linv and of
+// the ends moves a true end by a few ulps) and by 2^-1000.  A step whose x
+// stays in [lo, hi] changes neither index; one outside takes the rare branch,
+// which computes the indices.
+__device__ __forceinline__ void cell_bounds(const RunConst& c, double periodic, double well, double& lo, double& hi)
+{
+    const double a = fmax((periodic - 0.5) * c.L, well * c.L);
+    const double b = fmin((periodic + 0.5) * c.L, (well + 1.0) * c.L);
+    lo = a + (fabs(a) * 0x1p-48 + 0x1p-1000);
+    hi = b - (fabs(b) * 0x1p-48 + 0x1p-1000);
 }
 
 // InitializePeriodicIndices / UpdatePeriodicIndices (Single.h:1838-1884; Double.h:2150-2235)
@@ -1163,8 +1181,19 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     // only their crossings take the rare branch.
     int dead = active ? 0 : 1;
     uint64_t live_m = __ballot(active);
-    // single barrier: (periodic index) x L of the cell the coming step starts in
+    // single barrier: (periodic index) x L of the cell the coming step starts in,
+    // and the positions [cell_lo, cell_hi] certainly inside that cell (cell_bounds)
     double pL = s.periodic * c.L;
+    double cell_lo = 0.0, cell_hi = 0.0;
+    // The single barrier's well and periodic indices change only on the rare
+    // branch: they wait in LDS between its visits, so that no register carries
+    // them through the common step.
+    __shared__ double lane_cell[2][256];
+    if (f.model == 0) {
+        cell_bounds(c, s.periodic, s.well, cell_lo, cell_hi);
+        lane_cell[0][threadIdx.x] = s.well;
+        lane_cell[1][threadIdx.x] = s.periodic;
+    }
     // stores the state after step `last`, with `counted` steps of the launch in the state counts
     auto finish = [&](uint64_t last, uint64_t counted) __attribute__((always_inline)) {
         s.state = st ? 1.0 : 0.0;
@@ -1206,7 +1235,9 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
             s.v = c.expmgdt * s.v + c.var * g;
             s.x = s.x + s.v * c.halfdt;
         }
-        lane_indices(s, c, f, false);
+        // the single barrier's indices are computed on the rare branch, which a
+        // step leaving [cell_lo, cell_hi] takes (below)
+        if (f.model != 0) lane_indices(s, c, f, false);
         // per-step flip of the barrier under the particle (Single.h:2010-2015, 2078-2086);
         // equally probable states share one threshold, below 2^31 (p <= 1/2)
         if (!f.tau0) {
@@ -1226,11 +1257,19 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
         double F = f.model == 0 ? force_single_xp(s.x - pL, c.w, st ? sp.S[1] : sp.S[0], st ? sp.Sm[1] : sp.Sm[0])
                                 : lane_force(s, c, sp, f, st != 0);
         c1 += (uint32_t)st;
-        const uint64_t m_period = lanes_ne(s.periodic, s.periodic_prev);
-        const uint64_t m_rare = (m_period | lanes_ne(s.well, s.well_prev) |
-                                 (f.model == 0 ? 0ull : lanes_ne(s.local, s.local_prev))) &
-                                live_m;
+        const uint64_t m_rare =
+            (f.model == 0 ? (lanes_ult(s.x, cell_lo) | lanes_ugt(s.x, cell_hi))
+                          : (lanes_ne(s.periodic, s.periodic_prev) | lanes_ne(s.well, s.well_prev) |
+                             lanes_ne(s.local, s.local_prev))) &
+            live_m;
         if (__builtin_expect(m_rare != 0, 0)) {
+            if (f.model == 0) {
+                s.well = lane_cell[0][threadIdx.x];
+                s.periodic = lane_cell[1][threadIdx.x];
+                lane_indices(s, c, f, false);
+                lane_cell[0][threadIdx.x] = s.well;
+                lane_cell[1][threadIdx.x] = s.periodic;
+            }
             const uint64_t iStep = istep();
             const bool live = (dead == 0);
             const bool entering = f.model == 0 ? s.ent_period : s.ent_local;
@@ -1255,7 +1294,10 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
                 if (err) fail(iStep, iStep - s0);
             }
             live_m = __ballot(dead == 0);
-            if (f.model == 0) pL = s.periodic * c.L;
+            if (f.model == 0) {
+                pL = s.periodic * c.L;
+                cell_bounds(c, s.periodic, s.well, cell_lo, cell_hi);
+            }
         }
         s.F = F;
         if (!f.over) s.v = s.v + (c.minv * s.F) * c.halfdt;
@@ -1328,37 +1370,54 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
             // instead of being copied.  A stage array has two rows beyond the stage's
             // steps, so the reads one and two rows ahead stay inside it; those rows
             // are never used.
-            auto stage_steps = [&](uint4 (&buf)[kStageSlots], uint64_t base, uint32_t m, auto frames)
+            // rows r0 .. m - 1 of the stage (steps base + r0 + 1 .. base + m)
+            auto stage_steps = [&](uint4 (&buf)[kStageSlots], uint64_t base, uint32_t r0, uint32_t m)
                                    __attribute__((always_inline)) {
-                const uint4* rec = &buf[slot];
+                const uint4* rec = &buf[slot + r0 * S];
                 uint4 ra = rec[0];
-                const uint32_t pairs = m >> 1;
+                const uint32_t n = m - r0;
+                const uint64_t b0 = base + r0;
+                const uint32_t pairs = n >> 1;
                 for (uint32_t k = 0; k < pairs; ++k) {
-                    // rows 2k + 1 and 2k + 2 (the last read of an even stage is padding row m)
+                    // rows r0 + 2k and r0 + 2k + 1 (the last read of an even run is the row
+                    // after it: padding row m at most)
                     const uint4 rb = rec[S];
                     double g;
                     uint32_t fw;
                     buffered_noise(ra, g, fw);
-                    step([&] { return base + 2 * k + 1; }, g, fw, eq, frames);
+                    step([&] { return b0 + 2 * k + 1; }, g, fw, eq, std::false_type{});
                     ra = rec[2 * S];
                     buffered_noise(rb, g, fw);
-                    step([&] { return base + 2 * k + 2; }, g, fw, eq, frames);
+                    step([&] { return b0 + 2 * k + 2; }, g, fw, eq, std::false_type{});
                     rec += 2 * S;
                 }
-                if (m & 1) {
+                if (n & 1) {
                     double g;
                     uint32_t fw;
                     buffered_noise(ra, g, fw);
-                    step([&] { return base + m; }, g, fw, eq, frames);
+                    step([&] { return base + m; }, g, fw, eq, std::false_type{});
                 }
             };
-            // a stage holding a frame step runs with the frame test, the others without
+            // A stage runs without a frame test: its steps are split at the frame
+            // steps it holds, each frame recorded after its step.
             auto run_stage = [&](uint4 (&buf)[kStageSlots], uint64_t q) __attribute__((always_inline)) {
                 const uint64_t first = q * L;
                 const uint32_t m = (uint32_t)(len - first < L ? len - first : L);
                 const uint64_t base = uniform_u64(s0 + first);
-                if (next_save <= base + m) stage_steps(buf, base, m, std::true_type{});
-                else stage_steps(buf, base, m, std::false_type{});
+                uint32_t r = 0;
+                while (next_save <= base + m) {
+                    const uint32_t e = (uint32_t)(next_save - base);
+                    stage_steps(buf, base, r, e);
+                    if (dead == 0) {
+                        s.state = st ? 1.0 : 0.0;
+                        lane_param(s, sp, f, false);
+                        record(c, f, P, t, s, frame);
+                    }
+                    frame++;
+                    next_save += c.save_freq;
+                    r = e;
+                }
+                if (r < m) stage_steps(buf, base, r, m);
             };
             // stage q has landed (and every earlier LDS read is done): load stage q + 1
             // into the other array, whose stage q - 1 has been read
@@ -1410,6 +1469,10 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     // one copy of the loops per flip rule (uniform: a kernel argument)
     if (c.equal_prob) run(std::true_type{});
     else run(std::false_type{});
+    if (f.model == 0) {
+        s.well = lane_cell[0][threadIdx.x];
+        s.periodic = lane_cell[1][threadIdx.x];
+    }
     if (dead == 0) finish(s1, s1 - s0);
     if constexpr (kLocal) {
         if (active) {
