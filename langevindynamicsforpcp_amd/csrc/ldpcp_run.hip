@@ -228,10 +228,9 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v)
     return ((uint64_t)hi << 32) | lo;
 }
 
-// The lanes where a != b: the compare's lane mask itself (one v_cmp; a ballot of
-// a bool would first spell the bool out in a vector register).
-__device__ __forceinline__ uint64_t lanes_ne(double a, double b) { return __builtin_amdgcn_fcmp(a, b, 14); }
-// the lanes where a < b, and where a > b, or either is NaN
+// The lanes where a < b, and where a > b, or either is NaN: the compare's lane
+// mask itself (one v_cmp; a ballot of a bool would first spell the bool out in
+// a vector register).
 __device__ __forceinline__ uint64_t lanes_ult(double a, double b) { return __builtin_amdgcn_fcmp(a, b, 12); }
 __device__ __forceinline__ uint64_t lanes_ugt(double a, double b) { return __builtin_amdgcn_fcmp(a, b, 10); }
 
@@ -413,19 +412,60 @@ __device__ __forceinline__ double lane_force(const Lane& s, const RunConst& c, c
     return F;
 }
 
-// Positions certainly inside the single barrier's cell of (periodic, well),
-// where rint(x Linv) == periodic and floor(x Linv) == well (lane_indices):
-// the cell's ends (periodic -+ 1/2) L, well L and (well + 1) L moved inwards by
-// 2^-48 of their magnitude (at least 16 ulps; the rounding of x Linv and of
+// Positions certainly inside a lane's cell: where rint(x Linv) == periodic,
+// floor(x Linv) == well and, for the double barrier, (x - periodic L < xc) ==
+// (local == 0) (lane_indices).  The cell's ends (periodic -+ 1/2) L, well L,
+// (well + 1) L and periodic L + xc are moved inwards by 2^-48 of their
+// magnitude (at least 16 ulps; the rounding of x Linv, x - periodic L and of
 // the ends moves a true end by a few ulps) and by 2^-1000.  A step whose x
-// stays in [lo, hi] changes neither index; one outside takes the rare branch,
-// which computes the indices.
-__device__ __forceinline__ void cell_bounds(const RunConst& c, double periodic, double well, double& lo, double& hi)
+// stays in [lo, hi] changes no index; one outside takes the rare branch, which
+// computes the indices.
+__device__ __forceinline__ void cell_bounds(const RunConst& c, const Flags& f, double periodic, double well,
+                                            double local, double& lo, double& hi)
 {
-    const double a = fmax((periodic - 0.5) * c.L, well * c.L);
-    const double b = fmin((periodic + 0.5) * c.L, (well + 1.0) * c.L);
+    double a = fmax((periodic - 0.5) * c.L, well * c.L);
+    double b = fmin((periodic + 0.5) * c.L, (well + 1.0) * c.L);
+    if (f.model == 1) {
+        const double xm = periodic * c.L + c.xc;
+        a = local == 0.0 ? a : fmax(a, xm);
+        b = local == 0.0 ? fmin(b, xm) : b;
+    }
     lo = a + (fabs(a) * 0x1p-48 + 0x1p-1000);
     hi = b - (fabs(b) * 0x1p-48 + 0x1p-1000);
+}
+
+// The double barrier's force inside the barrier of a lane's local half
+// (local == 0: barrier 0, whose segments are (lo[0], xb[0]) and (xb[0], xc);
+// local == 1: barrier 1, (xc, xb[1]) and (xb[1], hi[1])): lane_force's four
+// segment tests reduced to the two that can hold there.  a and b are the
+// segments' forces 0 - slope, [state].
+struct LocalBarrier {
+    double lo, xb, hi, a[2], b[2];
+};
+
+__device__ __forceinline__ LocalBarrier local_barrier(const RunConst& c, const StateParams& sp, double local)
+{
+    const bool one = local != 0.0;
+    LocalBarrier r;
+    r.lo = one ? c.xc : c.lo[0];
+    r.xb = one ? c.xb[1] : c.xb[0];
+    r.hi = one ? c.hi[1] : c.xc;
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+        r.a[st] = one ? sp.dfn[st][1][0] : sp.dfn[st][0][0];
+        r.b[st] = one ? sp.dfn[st][1][1] : sp.dfn[st][0][1];
+    }
+    return r;
+}
+
+__device__ __forceinline__ double force_double_local(double xp, const LocalBarrier& lb, bool st)
+{
+    const bool n = (xp < lb.xb) && (xp > lb.lo);
+    const bool p = (xp < lb.hi) && (xp > lb.xb);
+    double F = 0.0;
+    F = n ? (st ? lb.a[1] : lb.a[0]) : F;
+    F = p ? (st ? lb.b[1] : lb.b[0]) : F;
+    return F;
 }
 
 // InitializePeriodicIndices / UpdatePeriodicIndices (Single.h:1838-1884; Double.h:2150-2235)
@@ -1181,19 +1221,21 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     // only their crossings take the rare branch.
     int dead = active ? 0 : 1;
     uint64_t live_m = __ballot(active);
-    // single barrier: (periodic index) x L of the cell the coming step starts in,
-    // and the positions [cell_lo, cell_hi] certainly inside that cell (cell_bounds)
+    // (periodic index) x L of the cell the coming step starts in, the positions
+    // [cell_lo, cell_hi] certainly inside that cell (cell_bounds) and, for the
+    // double barrier, the barrier of its local half
     double pL = s.periodic * c.L;
     double cell_lo = 0.0, cell_hi = 0.0;
-    // The single barrier's well and periodic indices change only on the rare
-    // branch: they wait in LDS between its visits, so that no register carries
-    // them through the common step.
-    __shared__ double lane_cell[2][256];
-    if (f.model == 0) {
-        cell_bounds(c, s.periodic, s.well, cell_lo, cell_hi);
-        lane_cell[0][threadIdx.x] = s.well;
-        lane_cell[1][threadIdx.x] = s.periodic;
-    }
+    cell_bounds(c, f, s.periodic, s.well, s.local, cell_lo, cell_hi);
+    LocalBarrier lb{};
+    if (f.model == 1) lb = local_barrier(c, sp, s.local);
+    // The well, periodic and local indices change only on the rare branch: they
+    // wait in LDS between its visits, so that no register carries them through
+    // the common step.
+    __shared__ double lane_cell[MODEL == 1 ? 3 : 2][256];
+    lane_cell[0][threadIdx.x] = s.well;
+    lane_cell[1][threadIdx.x] = s.periodic;
+    if constexpr (MODEL == 1) lane_cell[2][threadIdx.x] = s.local;
     // stores the state after step `last`, with `counted` steps of the launch in the state counts
     auto finish = [&](uint64_t last, uint64_t counted) __attribute__((always_inline)) {
         s.state = st ? 1.0 : 0.0;
@@ -1235,9 +1277,8 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
             s.v = c.expmgdt * s.v + c.var * g;
             s.x = s.x + s.v * c.halfdt;
         }
-        // the single barrier's indices are computed on the rare branch, which a
-        // step leaving [cell_lo, cell_hi] takes (below)
-        if (f.model != 0) lane_indices(s, c, f, false);
+        // the indices are computed on the rare branch, which a step leaving
+        // [cell_lo, cell_hi] takes (below)
         // per-step flip of the barrier under the particle (Single.h:2010-2015, 2078-2086);
         // equally probable states share one threshold, below 2^31 (p <= 1/2)
         if (!f.tau0) {
@@ -1251,25 +1292,23 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
             }
             st = flip ? st ^ 1 : st;
         }
-        // the single barrier's offset from the origin of the cell the step started
-        // in (pL): exact while the periodic index is unchanged, and a step that
-        // changes it takes the rare branch, which recomputes F from x
-        double F = f.model == 0 ? force_single_xp(s.x - pL, c.w, st ? sp.S[1] : sp.S[0], st ? sp.Sm[1] : sp.Sm[0])
-                                : lane_force(s, c, sp, f, st != 0);
+        // the force from the offset to the origin of the cell the step started in
+        // (pL) and, for the double barrier, that cell's local barrier: exact while
+        // the step stays in the cell; a step that leaves it takes the rare branch,
+        // which recomputes F from x where the periodic or local index changed
+        const double xp = s.x - pL;
+        double F = f.model == 0 ? force_single_xp(xp, c.w, st ? sp.S[1] : sp.S[0], st ? sp.Sm[1] : sp.Sm[0])
+                                : force_double_local(xp, lb, st != 0);
         c1 += (uint32_t)st;
-        const uint64_t m_rare =
-            (f.model == 0 ? (lanes_ult(s.x, cell_lo) | lanes_ugt(s.x, cell_hi))
-                          : (lanes_ne(s.periodic, s.periodic_prev) | lanes_ne(s.well, s.well_prev) |
-                             lanes_ne(s.local, s.local_prev))) &
-            live_m;
+        const uint64_t m_rare = (lanes_ult(s.x, cell_lo) | lanes_ugt(s.x, cell_hi)) & live_m;
         if (__builtin_expect(m_rare != 0, 0)) {
-            if (f.model == 0) {
-                s.well = lane_cell[0][threadIdx.x];
-                s.periodic = lane_cell[1][threadIdx.x];
-                lane_indices(s, c, f, false);
-                lane_cell[0][threadIdx.x] = s.well;
-                lane_cell[1][threadIdx.x] = s.periodic;
-            }
+            s.well = lane_cell[0][threadIdx.x];
+            s.periodic = lane_cell[1][threadIdx.x];
+            if constexpr (MODEL == 1) s.local = lane_cell[2][threadIdx.x];
+            lane_indices(s, c, f, false);
+            lane_cell[0][threadIdx.x] = s.well;
+            lane_cell[1][threadIdx.x] = s.periodic;
+            if constexpr (MODEL == 1) lane_cell[2][threadIdx.x] = s.local;
             const uint64_t iStep = istep();
             const bool live = (dead == 0);
             const bool entering = f.model == 0 ? s.ent_period : s.ent_local;
@@ -1294,10 +1333,9 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
                 if (err) fail(iStep, iStep - s0);
             }
             live_m = __ballot(dead == 0);
-            if (f.model == 0) {
-                pL = s.periodic * c.L;
-                cell_bounds(c, s.periodic, s.well, cell_lo, cell_hi);
-            }
+            pL = s.periodic * c.L;
+            cell_bounds(c, f, s.periodic, s.well, s.local, cell_lo, cell_hi);
+            if (f.model == 1) lb = local_barrier(c, sp, s.local);
         }
         s.F = F;
         if (!f.over) s.v = s.v + (c.minv * s.F) * c.halfdt;
@@ -1305,6 +1343,7 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
             if (istep() == next_save) {
                 if (dead == 0) {
                     s.state = st ? 1.0 : 0.0;
+                    s.xp = s.x - pL;  // the double barrier's energy reads it
                     lane_param(s, sp, f, false);
                     record(c, f, P, t, s, frame);
                 }
@@ -1410,6 +1449,7 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
                     stage_steps(buf, base, r, e);
                     if (dead == 0) {
                         s.state = st ? 1.0 : 0.0;
+                        s.xp = s.x - pL;  // the double barrier's energy reads it
                         lane_param(s, sp, f, false);
                         record(c, f, P, t, s, frame);
                     }
@@ -1469,9 +1509,11 @@ __global__ void __launch_bounds__(256) run_step_kernel(RunConst c, RunPtrs P, ui
     // one copy of the loops per flip rule (uniform: a kernel argument)
     if (c.equal_prob) run(std::true_type{});
     else run(std::false_type{});
-    if (f.model == 0) {
-        s.well = lane_cell[0][threadIdx.x];
-        s.periodic = lane_cell[1][threadIdx.x];
+    s.well = lane_cell[0][threadIdx.x];
+    s.periodic = lane_cell[1][threadIdx.x];
+    if constexpr (MODEL == 1) {
+        s.local = lane_cell[2][threadIdx.x];
+        s.xp = s.x - s.periodic * c.L;
     }
     if (dead == 0) finish(s1, s1 - s0);
     if constexpr (kLocal) {
