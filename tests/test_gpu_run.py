@@ -128,25 +128,32 @@ def test_gpu_barrier_window_overflow_is_an_error():
 def test_gpu_buffered_noise_equals_inline_noise(name):
     """A handle of few trajectories draws its Philox noise in a separate kernel
     (staged through LDS); a large handle draws it inside the step loop.  The
-    trajectories both handles share (same ids) must agree bit for bit."""
+    trajectories both handles share (same ids) must agree bit for bit.  The
+    buffered handles cover a one-wavefront handle of 7 trajectories (stages of
+    308 steps, lanes 7-63 dead), one of 64 (32-step stages) and one of 200
+    (four wavefronts loading row by row, the last with 8 trajectories)."""
     p = make(name, nsteps=40000, save_freq=100, save_all=1)
-    small, big = 64, 8192 + 128          # below / above the buffered-noise limit
+    big = 8192 + 128                     # above the buffered-noise limit
     print(f"[gpu noise paths] {name}", flush=True)
-    a = to_engine(p, small, seed=31)
-    a.tick_capacity = 256
-    a.steps_per_launch = 3001            # stages of 32 steps straddle launches
-    a.Integrate(0)
     b = to_engine(p, big, seed=31)
     b.tick_capacity = 256
     b.Integrate(0)
-    la, lb = a.lanes(), b.lanes()
-    for t in range(small):
-        for f, _ in type(la[t])._fields_:
-            x, y = getattr(la[t], f), getattr(lb[t], f)
-            assert (list(x) if hasattr(x, "__len__") else x) == (list(y) if hasattr(y, "__len__") else y), (t, f)
-    for field in range(7):
-        np.testing.assert_array_equal(a.frames(field), b.frames(field)[:small])
-    a.close()
+    lb = b.lanes()
+    for small in (7, 64, 200):
+        print(f"[gpu noise paths] {name} buffered handle of {small}", flush=True)
+        a = to_engine(p, small, seed=31)
+        a.tick_capacity = 256
+        a.steps_per_launch = 3001        # stages straddle launches
+        a.Integrate(0)
+        la = a.lanes()
+        for t in range(small):
+            for f, _ in type(la[t])._fields_:
+                x, y = getattr(la[t], f), getattr(lb[t], f)
+                assert (list(x) if hasattr(x, "__len__") else x) == (list(y) if hasattr(y, "__len__") else y), \
+                    (small, t, f)
+        for field in range(7):
+            np.testing.assert_array_equal(a.frames(field), b.frames(field)[:small])
+        a.close()
     b.close()
 
 
