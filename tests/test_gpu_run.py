@@ -124,7 +124,8 @@ def test_gpu_barrier_window_overflow_is_an_error():
     le.close()
 
 
-@pytest.mark.parametrize("name", ["single_over", "single_under", "double_over", "double_under", "single_tau0"])
+@pytest.mark.parametrize("name", ["single_over", "single_under", "double_over", "double_under", "single_tau0",
+                                  "double_tau0"])
 def test_gpu_buffered_noise_equals_inline_noise(name):
     """A handle of few trajectories draws its Philox noise in a separate kernel
     (staged through LDS); a large handle draws it inside the step loop.  The

@@ -36,6 +36,9 @@ CONFIGS = {
     "double_under": dict(model=1, overdamped=0, x0=-2.9, periodic_length=5.8,
                          heights=[[1.0, 0.5], [2.0, 0.5]], half_width=[0.70, 0.65], position=[-1.42, -0.56],
                          timestep=2e-3, tau=1e-2, randomize_v=1, gamma=3.0, temperature=2.0),
+    "double_tau0": dict(model=1, overdamped=1, x0=-2.9, periodic_length=5.8,
+                        heights=[[1.0, 0.5], [2.0, 0.5]], half_width=[0.70, 0.65], position=[-1.42, -0.56],
+                        timestep=2e-3, tau=0.0, randomize_v=0),
 }
 
 
