@@ -572,7 +572,7 @@ def main():
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X; "
+        print("bench.py measures one GPU: the library drives a single MI355X; "
               "--gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
 

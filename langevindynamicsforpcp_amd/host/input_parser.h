@@ -43,7 +43,7 @@ public:
     std::vector<std::string> GetOptionValue(const std::string& key, int index = 0) const;
     size_t GetNumberOfOptionValue(const std::string& key) const;
     bool GetOptionSpecified(const std::string& key) const;
-    // options this build adds beyond the reference's -i/-o/-p
+    // options beyond the reference's -i/-o/-p: the emulated ranks
     int ranks = 1;             // --ranks N: emulated MPI ranks (default: PMI/OMPI world size or 1)
     int simd_width = 8;        // --simd-width W: trajectories per rank per trial
     unsigned long steps_per_launch = 1ul << 20;
