@@ -275,7 +275,7 @@ def serving(seconds, budget_gb, rate=20.0, seed=11, fp64_limit_s=30.0, engine_ra
                        "trajectories": "log-uniform in [1e3, 1e6]", "eb": "U[2, 6.5], ratio 3.6",
                        "cap_note": "cap 1e5 steps (100 time units at dt 1e-3): the stream offers at most "
                                    "2.9e6 trajectories/s x 1e5 steps = 2.9e11 trajectory-steps/s, under the FP32 "
-                                   "engine's rate; C2's 1e8 cap would offer ~400x the device's rate"},
+                                   "engine's rate; the 8-set sweep's 1e8 cap would offer ~400x the device's rate"},
            "absorbed_fraction": float(sum(r.n_absorbed for r in res["results"]) /
                                       max(sum(r.n_absorbed + r.n_capped for r in res["results"]), 1))}
     if engine_rate:

@@ -229,7 +229,7 @@ def test_ragged_batch_fp32_statistics_match_fp64():
     assert checked >= 4
 
 
-def test_reference_case_c1_matches_oracle(oracle):
+def test_underdamped_32_trajectories_match_oracle(oracle):
     """The reference's CPU-sized case: 32 trajectories (4 ranks x 8 lanes) of
     the single-barrier underdamped model (barrier 7.2 / 2.0, L = 5.8,
     w = 0.65, dt = 1e-4, tau = 1e-5), at rest in the left well centre, run
@@ -240,7 +240,7 @@ def test_reference_case_c1_matches_oracle(oracle):
     cap, frames_cap = 10 ** 7, 100001
     s = sweep_set(model=0, overdamped=False, eb=2.0, ratio=3.6, n_traj=32, max_steps=cap, seed=1, x0=-2.9,
                   periodic_length=5.8, half_width=(0.65, 0.65), timestep=1e-4, tau=1e-5, randomize_v=False, v0=0.0)
-    print("[gpu engine] reference case C1", flush=True)
+    print("[gpu engine] 32 underdamped trajectories, barrier 7.2/2.0", flush=True)
     (paths, st), = run_sweep([s], precision=64)
     ab, tp = paths[0], paths[1]
     p = ol.make_params(model=0, overdamped=0, x0=-2.9, v0=0.0, randomize_v=0, periodic_length=5.8,
